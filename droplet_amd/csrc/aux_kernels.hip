@@ -1,0 +1,233 @@
+// aux_kernels.hip — initialisation, moment recomputation, LES relaxation-time
+// field and diagnostics reductions (FP64).  None of these is on the per-step hot
+// path except the LES tau field, which replaces the eddy-viscosity part of the
+// reference `physics` (program/LES_drop_xy.f90:661-693).
+#include "d3q15.h"
+#include "kernels.h"
+
+namespace drp {
+
+template <int Q0, int Q1, typename Fn> __device__ __forceinline__ void sfor(Fn&& fn) {
+    if constexpr (Q0 < Q1) {
+        fn(std::integral_constant<int, Q0>{});
+        sfor<Q0 + 1, Q1>(fn);
+    }
+}
+
+struct Grid {
+    int nx, ny, nz;
+    __device__ __forceinline__ int64_t at(int x, int y, int z) const {
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+        return x + (int64_t)nx * (y + (int64_t)ny * z);
+    }
+};
+
+// lap phi and grad phi at (x,y,z) from global memory (various_cal :608-633)
+__device__ void phi_stencil(const Grid& G, const double* phi, int x, int y, int z, double& lap, double& g0,
+                            double& g1, double& g2) {
+    lap = -14.0 * phi[G.at(x, y, z)];
+    g0 = g1 = g2 = 0.0;
+    sfor<1, 15>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        using D = Dir<q>;
+        const double v = phi[G.at(x + D::cx, y + D::cy, z + D::cz)];
+        lap = lap + v;
+        g0 = acc<D::cx>(g0, v);
+        g1 = acc<D::cy>(g1, v);
+        g2 = acc<D::cz>(g2, v);
+    });
+    lap = lap / 5.0;
+    g0 = g0 / 10.0;
+    g1 = g1 / 10.0;
+    g2 = g2 / 10.0;
+}
+
+// grad u (gu[k][a] = d_a u_k) at (x,y,z) (various_cal :616-633)
+__device__ void u_stencil(const Grid& G, const double* u1, const double* u2, const double* u3, int x, int y, int z,
+                          double gu[3][3]) {
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) gu[k][a] = 0.0;
+    sfor<1, 15>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        using D = Dir<q>;
+        const int64_t t = G.at(x + D::cx, y + D::cy, z + D::cz);
+        const double v[3] = {u1[t], u2[t], u3[t]};
+        for (int k = 0; k < 3; k++) {
+            gu[k][0] = acc<D::cx>(gu[k][0], v[k]);
+            gu[k][1] = acc<D::cy>(gu[k][1], v[k]);
+            gu[k][2] = acc<D::cz>(gu[k][2], v[k]);
+        }
+    });
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) gu[k][a] = gu[k][a] / 10.0;
+}
+
+__global__ void init_equilibrium_fp64(Grid G, Phys P, const double* __restrict__ mac, double* __restrict__ f,
+                                      double* __restrict__ g) {
+    const int64_t N = (int64_t)G.nx * G.ny * G.nz;
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    const int x = s % G.nx, y = (s / G.nx) % G.ny, z = s / ((int64_t)G.nx * G.ny);
+    NodeEq<double> nd;
+    nd.phi = mac[s];
+    nd.p = mac[N + s];
+    nd.u1 = mac[2 * N + s];
+    nd.u2 = mac[3 * N + s];
+    nd.u3 = mac[4 * N + s];
+    double lap;
+    phi_stencil(G, mac, x, y, z, lap, nd.g0, nd.g1, nd.g2);
+    const double ph = nd.phi;
+    const double p0 = ph * P.T / (1.0 - P.b * ph) - P.a * (ph * ph);
+    nd.A = p0 - P.kf * ph * lap - P.kf6 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
+    const double usq = nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
+    const double s2h = 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
+    sfor<0, 15>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
+        f[q * N + s] = feq<q>(nd, gt, P.kf);
+        g[q * N + s] = geq<q>(nd, usq, gt, P.kg);
+    });
+}
+
+__global__ void moments_fp64(int64_t N, const double* __restrict__ f, const double* __restrict__ g,
+                             double* __restrict__ mac, int which) {
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    double ph = f[s], pr = g[s] / 3.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+    sfor<1, 15>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        using D = Dir<q>;
+        const double gv = g[q * N + s];
+        ph = ph + f[q * N + s];
+        pr = pr + gv / 3.0;
+        m1 = acc<D::cx>(m1, gv);
+        m2 = acc<D::cy>(m2, gv);
+        m3 = acc<D::cz>(m3, gv);
+    });
+    if (which & 1) mac[s] = ph;
+    if (which & 2) {
+        mac[N + s] = pr;
+        mac[2 * N + s] = m1;
+        mac[3 * N + s] = m2;
+        mac[4 * N + s] = m3;
+    }
+}
+
+__global__ void les_tau_fp64(Grid G, Phys P, const double* __restrict__ mac, const uint64_t* __restrict__ range,
+                             int slot, double* __restrict__ tau) {
+    const int64_t N = (int64_t)G.nx * G.ny * G.nz;
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= N) return;
+    const int x = s % G.nx, y = (s / G.nx) % G.ny, z = s / ((int64_t)G.nx * G.ny);
+    double gu[3][3];
+    u_stencil(G, mac + 2 * N, mac + 3 * N, mac + 4 * N, x, y, z, gu);
+    // S_ab = 0.5 (d_b u_a + d_a u_b), nu_e = Cs^2 Delta^2 sqrt(2 S:S)  (:671-675, strain :481-501)
+    double sum = 0.0;
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+            const double sab = 0.5 * (gu[a][b] + gu[b][a]);
+            sum = sum + sab * sab;
+        }
+    const double nue = P.les_c * sqrt(2.0 * sum);
+    const double rmax = from_ordered_bits(range[2 * slot]), rmin = from_ordered_bits(range[2 * slot + 1]);
+    const double rg = rmax - rmin;
+    double frac = (mac[s] - rmin) / rg;
+    if (rg == 0.0) frac = 0.0;
+    const double nu = frac * P.dnu + P.nu1;
+    tau[s] = 0.5 + 3.0 * (nu + nue);
+}
+
+// Diagnostics: per-workgroup partial sums, reduced in fixed order on the host.
+// [0] sum phi  [1] sum 0.5|u|^2  [2] sum (d_a u_b + d_b u_a)^2  [3] sum |grad phi|
+// [4] sum psi(phi)  [5] sum p  [6..8] sum u  [9] unused
+__global__ void reduce_fp64(Grid G, Phys P, const double* __restrict__ mac, double* __restrict__ partial) {
+    __shared__ double sh[NPART][4];
+    const int64_t N = (int64_t)G.nx * G.ny * G.nz;
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double v[NPART] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (s < N) {
+        const int x = s % G.nx, y = (s / G.nx) % G.ny, z = s / ((int64_t)G.nx * G.ny);
+        const double ph = mac[s], pr = mac[N + s];
+        const double u1 = mac[2 * N + s], u2 = mac[3 * N + s], u3 = mac[4 * N + s];
+        double lap, g0, g1, g2, gu[3][3];
+        phi_stencil(G, mac, x, y, z, lap, g0, g1, g2);
+        u_stencil(G, mac + 2 * N, mac + 3 * N, mac + 4 * N, x, y, z, gu);
+        double e = 0.0;
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) {
+                const double sab = gu[a][b] + gu[b][a];
+                e += sab * sab;
+            }
+        v[0] = ph;
+        v[1] = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+        v[2] = e;
+        v[3] = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        v[4] = ph * P.T * log(ph / (1.0 - P.b * ph)) - P.a * ph * ph;
+        v[5] = pr;
+        v[6] = u1;
+        v[7] = u2;
+        v[8] = u3;
+    }
+    for (int k = 0; k < NPART; k++) {
+        double t = v[k];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+        v[k] = t;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < NPART; k++) sh[k][wave] = v[k];
+    __syncthreads();
+    if (threadIdx.x < NPART) {
+        const int k = threadIdx.x;
+        partial[blockIdx.x * NPART + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+    }
+}
+
+__global__ void range_set(uint64_t* range, int slot, double vmax, double vmin) {
+    range[2 * slot] = ordered_bits(vmax);
+    range[2 * slot + 1] = ordered_bits(vmin);
+}
+
+// ------------------------------------------------------------------ launchers
+static inline int64_t nblk(int64_t n, int bs) { return (n + bs - 1) / bs; }
+
+hipError_t launch_init_equilibrium_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* f,
+                                        double* g, hipStream_t st) {
+    Grid G{nx, ny, nz};
+    const int64_t N = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(init_equilibrium_fp64, dim3(nblk(N, 256)), dim3(256), 0, st, G, ph, mac, f, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_moments_fp64(int nx, int ny, int nz, const double* f, const double* g, double* mac, int which,
+                               hipStream_t st) {
+    const int64_t N = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(moments_fp64, dim3(nblk(N, 256)), dim3(256), 0, st, N, f, g, mac, which);
+    return hipGetLastError();
+}
+
+hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const uint64_t* range,
+                               int slot, double* tau, hipStream_t st) {
+    Grid G{nx, ny, nz};
+    const int64_t N = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(les_tau_fp64, dim3(nblk(N, 256)), dim3(256), 0, st, G, ph, mac, range, slot, tau);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* partial,
+                              int* nblocks, hipStream_t st) {
+    Grid G{nx, ny, nz};
+    const int64_t N = (int64_t)nx * ny * nz;
+    *nblocks = (int)nblk(N, 256);
+    hipLaunchKernelGGL(reduce_fp64, dim3(*nblocks), dim3(256), 0, st, G, ph, mac, partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_range_set(uint64_t* range, int slot, double vmax, double vmin, hipStream_t st) {
+    hipLaunchKernelGGL(range_set, dim3(1), dim3(1), 0, st, range, slot, vmax, vmin);
+    return hipGetLastError();
+}
+
+}  // namespace drp

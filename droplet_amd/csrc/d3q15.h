@@ -1,0 +1,147 @@
+// d3q15.h — D3Q15 lattice constants and per-direction equilibrium / collision
+// arithmetic shared by the HIP kernels.
+//
+// Every expression keeps the evaluation order of the reference Fortran
+// (program/lbm_drop_xy.f90:421-515, :541-553; prototype_program/two-phase/
+// dropch.f90:666-698), with the terms that are identically zero for a given
+// direction left out (adding +-0 to a non-zero partial sum leaves it unchanged),
+// so that with FMA contraction off the FP64 kernels round exactly like the
+// reference.  Directions are compile-time template arguments: the zero terms
+// vanish at compile time instead of costing multiplies.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace drp {
+
+// q = 0..14  <->  reference i = 1..15   (program/lbm_drop_xy.f90:70-84)
+template <int Q> struct Dir;
+#define DRP_DIR(Q, X, Y, Z, O)                                             \
+    template <> struct Dir<Q> {                                            \
+        static constexpr int cx = X, cy = Y, cz = Z, opp = O;              \
+    };
+DRP_DIR(0, 0, 0, 0, 0)
+DRP_DIR(1, 1, 0, 0, 4)
+DRP_DIR(2, 0, 1, 0, 5)
+DRP_DIR(3, 0, 0, 1, 6)
+DRP_DIR(4, -1, 0, 0, 1)
+DRP_DIR(5, 0, -1, 0, 2)
+DRP_DIR(6, 0, 0, -1, 3)
+DRP_DIR(7, 1, 1, 1, 11)
+DRP_DIR(8, -1, 1, 1, 12)
+DRP_DIR(9, 1, -1, 1, 13)
+DRP_DIR(10, 1, 1, -1, 14)
+DRP_DIR(11, -1, -1, -1, 7)
+DRP_DIR(12, 1, -1, -1, 8)
+DRP_DIR(13, -1, 1, -1, 9)
+DRP_DIR(14, -1, -1, 1, 10)
+#undef DRP_DIR
+
+template <int Q> __host__ __device__ constexpr double weightE() {
+    return Q == 0 ? 2.0 / 9.0 : (Q <= 6 ? 1.0 / 9.0 : 1.0 / 72.0);
+}
+template <int Q> __host__ __device__ constexpr double weightF() {
+    return Q == 0 ? -7.0 / 3.0 : (Q <= 6 ? 1.0 / 3.0 : 1.0 / 24.0);
+}
+
+// signed accumulation helper: s (+|-) v, or s unchanged when c == 0
+template <int C, typename T> __device__ __forceinline__ T acc(T s, T v) {
+    if constexpr (C == 1) return s + v;
+    else if constexpr (C == -1) return s - v;
+    else return s;
+}
+// first term of a signed sum started from zero: 0 + c v
+template <int C, typename T> __device__ __forceinline__ T first(T v) {
+    if constexpr (C == 1) return v;
+    else if constexpr (C == -1) return -v;
+    else return T(0);
+}
+
+// c . v with the reference's order ((cx v1 + cy v2) + cz v3), zero terms dropped
+template <int Q, typename T> __device__ __forceinline__ T cdot(T v1, T v2, T v3) {
+    using D = Dir<Q>;
+    if constexpr (D::cx != 0) {
+        T s = first<D::cx>(v1);
+        s = acc<D::cy>(s, v2);
+        return acc<D::cz>(s, v3);
+    } else if constexpr (D::cy != 0) {
+        T s = first<D::cy>(v2);
+        return acc<D::cz>(s, v3);
+    } else if constexpr (D::cz != 0) {
+        return first<D::cz>(v3);
+    } else {
+        return T(0);
+    }
+}
+
+// Korteweg contraction gt = sum_b sum_a G_ab c_a c_b (beta outer, alpha inner,
+// equilibrium_cal :439-449) with G_ab = (4.5 g_a) g_b - (1.5 s2) delta_ab
+// (gphi_cal :642-666) and s2 = (g0^2 + g1^2) + g2^2.
+template <int Q, typename T>
+__device__ __forceinline__ T korteweg(T g0, T g1, T g2, T s2h /* 1.5*s2 */) {
+    using D = Dir<Q>;
+    constexpr int c[3] = {D::cx, D::cy, D::cz};
+    const T g[3] = {g0, g1, g2};
+    if constexpr (Q == 0) {
+        return T(0);
+    } else if constexpr (Q <= 6) {
+        constexpr int k = D::cx != 0 ? 0 : (D::cy != 0 ? 1 : 2);
+        return T(4.5) * g[k] * g[k] - s2h;
+    } else {
+        T s = T(4.5) * g[0] * g[0] - s2h;                      // (0,0)
+#pragma unroll
+        for (int t = 1; t < 9; t++) {                           // (a,b) = (t%3, t/3)
+            const int a = t % 3, b = t / 3;
+            T G = (a == b) ? (T(4.5) * g[a] * g[b] - s2h) : (T(4.5) * g[a] * g[b]);
+            s = (c[a] * c[b] > 0) ? s + G : s - G;
+        }
+        return s;
+    }
+}
+
+// Per-node quantities an upstream equilibrium needs.
+template <typename T> struct NodeEq {
+    T phi, p, u1, u2, u3;
+    T A;          // F-bracket p0 - kf phi lap - kf/6 |grad phi|^2 (:453-455)
+    T g0, g1, g2; // grad phi
+    T tau;        // momentum relaxation time tau_g
+};
+
+// feq_q (:452-459): ((H phi + F A) + ((E phi) 3) cu) + (E kf) gt
+template <int Q, typename T> __device__ __forceinline__ T feq(const NodeEq<T>& n, T gt, T kf) {
+    constexpr T E = T(weightE<Q>()), Fw = T(weightF<Q>());
+    T s = (Q == 0) ? (n.phi + Fw * n.A) : (Fw * n.A);
+    if constexpr (Q != 0) {
+        T cu = cdot<Q>(n.u1, n.u2, n.u3);
+        s = s + E * n.phi * T(3.0) * cu;
+        s = s + E * kf * gt;
+    }
+    return s;
+}
+
+// geq_q (:461-469): E (((3p + 3cu) + 4.5 cu^2) - 1.5 |u|^2) + (E kg) gt   (Anu = 0)
+template <int Q, typename T> __device__ __forceinline__ T geq(const NodeEq<T>& n, T usq, T gt, T kg) {
+    constexpr T E = T(weightE<Q>());
+    if constexpr (Q == 0) {
+        return E * (T(3.0) * n.p - T(1.5) * usq);
+    } else {
+        T cu = cdot<Q>(n.u1, n.u2, n.u3);
+        T s = T(3.0) * n.p + T(3.0) * cu;
+        s = s + T(4.5) * (cu * cu);
+        s = s - T(1.5) * usq;
+        return E * s + E * kg * gt;
+    }
+}
+
+// order-preserving map of doubles to uint64 (for atomic min/max of the phi range)
+__host__ __device__ inline uint64_t ordered_bits(double v) {
+    uint64_t u = *reinterpret_cast<uint64_t*>(&v);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+__host__ __device__ inline double from_ordered_bits(uint64_t u) {
+    uint64_t r = (u & 0x8000000000000000ull) ? (u & 0x7fffffffffffffffull) : ~u;
+    return *reinterpret_cast<double*>(&r);
+}
+
+}  // namespace drp

@@ -1,0 +1,172 @@
+// deformation.cpp — Taylor deformation D = (L-B)/(L+B) of a sheared droplet.
+//
+// The reference evaluates it on rank 0 after gathering phi
+// (prototype_program/two-phase/dropch.f90:857-948, break_up3); here the one or
+// two phi planes it reads are copied off the device and the same serial
+// procedure runs on the host (it touches O(n^2) values once per call).
+//
+// Plane convention: the reference's walls are normal to z and the plane is
+// (x, z) at y = ymax/2 (centroid) and y = (ymax+1)/2 (search).  With walls
+// normal to y the roles of y and z swap.  Off-plane indices wrap periodically
+// (the reference reads its stale 1-cell halo there).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "lattice_impl.h"
+
+namespace {
+
+struct Plane {
+    std::vector<double> v;
+    int n1 = 0, n2 = 0;
+    double at(int i, int j) const {
+        i %= n1;
+        j %= n2;
+        if (i < 0) i += n1;
+        if (j < 0) j += n2;
+        return v[(size_t)j * n1 + i];
+    }
+    // bilinear interpolation on the unit cell [int(x), int(x)+1] x [int(z), int(z)+1] (:908-919)
+    double bilinear(double xj, double zj) const {
+        const int x1 = (int)xj, z1 = (int)zj, x2 = x1 + 1, z2 = z1 + 1;
+        const double w11 = ((double)x2 - xj) * ((double)z2 - zj), w21 = (xj - (double)x1) * ((double)z2 - zj);
+        const double w12 = ((double)x2 - xj) * (zj - (double)z1), w22 = (xj - (double)x1) * (zj - (double)z1);
+        return (w11 * at(x1, z1) + w21 * at(x2, z1) + w12 * at(x1, z2) + w22 * at(x2, z2)) /
+               (((double)x2 - (double)x1) * ((double)z2 - (double)z1));
+    }
+};
+
+}  // namespace
+
+int copy_phi_plane(drp_lattice* L, int third, double* host, int* n1, int* n2) {
+    const drp_config& c = L->cfg;
+    const bool wall_y = c.wall_axis == DRP_WALL_Y;
+    *n1 = c.nx;
+    *n2 = wall_y ? c.ny : c.nz;
+    const double* phi = nullptr;
+    if (c.precision == DRP_FP64) {
+        phi = (const double*)L->mac[L->cur];
+    } else {
+        int rc = fp32_macro_fp64_view(L, &phi);
+        if (rc) return rc;
+    }
+    hipError_t e;
+    if (wall_y) {  // plane (x, y) at z = third: contiguous
+        e = hipMemcpyAsync(host, phi + (size_t)third * c.nx * c.ny, sizeof(double) * c.nx * c.ny,
+                           hipMemcpyDeviceToHost, L->stream);
+    } else {       // plane (x, z) at y = third: one row of nx per z
+        e = hipMemcpy2DAsync(host, sizeof(double) * c.nx, phi + (size_t)third * c.nx, sizeof(double) * c.nx * c.ny,
+                             sizeof(double) * c.nx, c.nz, hipMemcpyDeviceToHost, L->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(L->stream);
+    if (e != hipSuccess) return drp_fail(DRP_ERR_HIP, "plane copy failed: %s", hipGetErrorString(e));
+    return DRP_OK;
+}
+
+static int load(drp_lattice* L, int third, Plane& P) {
+    const drp_config& c = L->cfg;
+    P.v.resize((size_t)c.nx * (c.wall_axis == DRP_WALL_Y ? c.ny : c.nz));
+    return copy_phi_plane(L, third, P.v.data(), &P.n1, &P.n2);
+}
+
+// break_up3 (dropch.f90:857-948)
+static int breakup3(drp_lattice* L, double* Lo, double* Bo, double* Do) {
+    const drp_config& c = L->cfg;
+    const int n3 = (c.wall_axis == DRP_WALL_Y) ? c.nz : c.ny;
+    const double pi = std::acos(-1.0), mid = 0.5 * (c.phi1 + c.phi2);
+    Plane C, S;
+    int rc = load(L, (n3 - 1) / 2, C);
+    if (!rc) rc = load(L, n3 / 2, S);
+    if (rc) return rc;
+    // centroid weighted by phi - phi1 (:862-877)
+    double xg = 0.0, zg = 0.0, w = 0.0;
+    for (int j = 0; j < C.n2; j++)
+        for (int i = 0; i < C.n1; i++) {
+            const double d = C.at(i, j) - c.phi1;
+            xg = xg + (double)i * d;
+            zg = zg + (double)j * d;
+            w = w + d;
+        }
+    xg = xg / w;
+    zg = zg / w;
+    // farthest interior point sets the orientation (:880-904)
+    double dist = 0.0;
+    int it = 0, jt = 0;
+    for (int j = 0; j < S.n2; j++)
+        for (int i = 0; i < S.n1; i++) {
+            if (S.at(i, j) < mid) continue;
+            const double r = std::sqrt(((double)i - xg) * ((double)i - xg) + ((double)j - zg) * ((double)j - zg));
+            if (dist <= r) {
+                dist = r;
+                it = i;
+                jt = j;
+            }
+        }
+    if (dist == 0.0) return drp_fail(DRP_ERR_STATE, "no interior point in the deformation plane");
+    double th = std::acos(std::fabs(((double)it - xg) / dist));
+    const double dx = (double)it - xg, dz = (double)jt - zg;
+    if (dx < 0.0 && dz > 0.0) th = pi - th;
+    else if (dx < 0.0 && dz < 0.0) th = pi + th;
+    else if (dx > 0.0 && dz < 0.0) th = 2.0 * pi - th;
+    // ray-march the minor and major half-axes in steps of 0.1 (:906-946)
+    const double den = std::tan(th) * std::sin(th) + std::cos(th);
+    double B1 = 0.0, L1 = 0.0;
+    for (int k = 1; k < 1000000; k++) {
+        double zj = (0.1 * (double)k) / den;
+        double xj = -zj * std::tan(th);
+        xj = xj + xg;
+        zj = zj + zg;
+        const double v = S.bilinear(xj, zj);
+        if (v < mid) break;
+        B1 = std::sqrt((xj - xg) * (xj - xg) + (zj - zg) * (zj - zg)) * 2.0;
+    }
+    for (int k = 1; k < 1000000; k++) {
+        double xj = (0.1 * (double)k) / den;
+        double zj = xj * std::tan(th);
+        xj = xj + xg;
+        zj = zj + zg;
+        const double v = S.bilinear(xj, zj);
+        if (v < mid) break;
+        L1 = std::sqrt((xj - xg) * (xj - xg) + (zj - zg) * (zj - zg)) * 2.0;
+    }
+    *Lo = L1;
+    *Bo = B1;
+    *Do = (L1 - B1) / (L1 + B1);
+    return DRP_OK;
+}
+
+// L = 2 max r, B = 4 A / (pi L) (prototype_program/two-phase/drop.f90:439-463)
+static int area_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
+    const drp_config& c = L->cfg;
+    const int n3 = (c.wall_axis == DRP_WALL_Y) ? c.nz : c.ny;
+    const double pi = std::acos(-1.0), mid = 0.5 * (c.phi1 + c.phi2);
+    Plane S;
+    int rc = load(L, n3 / 2, S);
+    if (rc) return rc;
+    double xg = 0.0, zg = 0.0, cnt = 0.0;
+    for (int j = 0; j < S.n2; j++)
+        for (int i = 0; i < S.n1; i++)
+            if (S.at(i, j) >= mid) {
+                xg += i;
+                zg += j;
+                cnt += 1.0;
+            }
+    if (cnt == 0.0) return drp_fail(DRP_ERR_STATE, "no interior point in the deformation plane");
+    xg /= cnt;
+    zg /= cnt;
+    double rmax = 0.0;
+    for (int j = 0; j < S.n2; j++)
+        for (int i = 0; i < S.n1; i++)
+            if (S.at(i, j) >= mid) rmax = std::fmax(rmax, std::sqrt((i - xg) * (i - xg) + (j - zg) * (j - zg)));
+    const double Lx = 2.0 * rmax, Bx = 4.0 * cnt / (pi * Lx);
+    *Lo = Lx;
+    *Bo = Bx;
+    *Do = (Lx - Bx) / (Lx + Bx);
+    return DRP_OK;
+}
+
+int deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do) {
+    return mode == 0 ? breakup3(L, Lo, Bo, Do) : area_variant(L, Lo, Bo, Do);
+}

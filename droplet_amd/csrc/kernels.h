@@ -1,0 +1,64 @@
+// kernels.h — launch interfaces between the C-ABI host code and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace drp {
+
+enum { FORCE_NONE = 0, FORCE_TAYLOR_GREEN = 1, FORCE_FIELD = 2 };
+enum { SHAPE_16x4x4 = 0, SHAPE_32x4x2 = 1, SHAPE_64x2x2 = 2, SHAPE_32x2x4 = 3, SHAPE_COUNT = 4 };
+
+// Physical constants of one lattice as the kernels need them.
+struct Phys {
+    double tau_f;     // phi-population relaxation time
+    double kf, kf6;   // kappa_f and kappa_f/6 (the reference folds kappaf/6.0d0 at compile time)
+    double kg;        // kappa_g
+    double T, a, b;   // van der Waals constants
+    double nu1, dnu;  // nu1 and nu2 - nu1
+    double phi1, phi2;
+    double tg_amp;    // Taylor-Green force amplitude
+    double les_c;     // Cs*Cs*Delta*Delta (evaluated left to right)
+    int les;          // Smagorinsky on
+};
+
+struct StepArgs {
+    int nx, ny, nz;
+    int tiles_x, tiles_y, tiles_z;
+    int64_t n;
+    const double* f_in;   // [15][n] phi populations (reference f)
+    const double* g_in;   // [15][n] momentum populations (reference g)
+    double* f_out;
+    double* g_out;
+    const double* mac_in; // phi, p, u1, u2, u3 (5 x n)
+    double* mac_out;
+    const double* tau_field;              // LES: tau_g per node, else nullptr
+    const double *fx, *fy, *fz;           // FORCE_FIELD
+    const double *sinx, *cosx, *sinz, *cosz;  // FORCE_TAYLOR_GREEN tables
+    double tau_f, kf, kf6, kg, T, a, b, nu1, dnu, tg_amp, phi1, phi2;
+    double wall_u_now;
+    int wall_axis;        // 0, 2 (y) or 3 (z)
+    int force_kind;
+    uint64_t* range;      // 3 slots x (max, min) as order-preserving bits
+    int slot_read, slot_write, slot_reset;
+};
+
+hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st);
+
+// populations := equilibria of the macroscopic fields (ini, program/lbm_drop_xy.f90:244-251)
+hipError_t launch_init_equilibrium_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* f,
+                                        double* g, hipStream_t st);
+// recompute moments from populations: which & 1 -> phi from f, which & 2 -> p, u from g
+hipError_t launch_moments_fp64(int nx, int ny, int nz, const double* f, const double* g, double* mac, int which,
+                               hipStream_t st);
+// Smagorinsky tau field (program/LES_drop_xy.f90:661-693) from u, phi and the phi range slot
+hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const uint64_t* range,
+                               int slot, double* tau, hipStream_t st);
+// diagnostics partial sums: NPART doubles per workgroup
+constexpr int NPART = 10;
+hipError_t launch_reduce_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* partial,
+                              int* nblocks, hipStream_t st);
+// fill a range slot with (max, min) = (vmax, vmin)
+hipError_t launch_range_set(uint64_t* range, int slot, double vmax, double vmin, hipStream_t st);
+
+}  // namespace drp

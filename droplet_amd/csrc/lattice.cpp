@@ -1,0 +1,448 @@
+// lattice.cpp — C ABI of include/droplet.h over the HIP kernels.
+//
+// Owns the device state of one lattice (populations ping-pong, macroscopic
+// fields ping-pong, the phi-range slots, optional LES tau / force fields) and
+// replays the reference loop body (program/lbm_drop_xy.f90:787-848) one kernel
+// launch per time step on the handle's stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/droplet.h"
+#include "d3q15.h"
+#include "kernels.h"
+#include "lattice_impl.h"
+
+using namespace drp;
+
+static thread_local std::string g_last_error;
+
+int drp_fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return drp_fail(DRP_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    } while (0)
+
+extern "C" const char* drp_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int drp_config_default(drp_config* c) {
+    if (!c) return drp_fail(DRP_ERR_ARG, "null config");
+    std::memset(c, 0, sizeof(*c));
+    c->nx = c->ny = c->nz = 64;
+    c->precision = DRP_FP64;
+    c->tau_f = 0.7;
+    c->nu1 = c->nu2 = 0.1;
+    c->kappa_f = 0.01;
+    c->kappa_g = 0.0;
+    c->phi1 = 2.211;
+    c->phi2 = 4.895;
+    c->a = 9.0 / 49.0;
+    c->b = 2.0 / 21.0;
+    c->T = 0.55;
+    c->wall_start = 1;
+    c->force_start = 1;
+    return DRP_OK;
+}
+
+Phys make_phys(const drp_config& c) {
+    Phys p;
+    p.tau_f = c.tau_f;
+    p.kf = c.kappa_f;
+    p.kf6 = c.kappa_f / 6.0;
+    p.kg = c.kappa_g;
+    p.T = c.T;
+    p.a = c.a;
+    p.b = c.b;
+    p.nu1 = c.nu1;
+    p.dnu = c.nu2 - c.nu1;
+    p.phi1 = c.phi1;
+    p.phi2 = c.phi2;
+    p.tg_amp = c.force_amp;
+    p.les_c = c.les_cs * c.les_cs * c.les_delta * c.les_delta;
+    p.les = (c.les_cs != 0.0 && c.les_delta != 0.0) ? 1 : 0;
+    return p;
+}
+
+static int validate(const drp_config* c) {
+    if (!c) return drp_fail(DRP_ERR_ARG, "null config");
+    if (c->nx < 3 || c->ny < 3 || c->nz < 3) return drp_fail(DRP_ERR_ARG, "grid must be at least 3^3");
+    if ((int64_t)c->nx * c->ny * c->nz * 15 > (int64_t)1 << 40) return drp_fail(DRP_ERR_ARG, "grid too large");
+    if (c->precision != DRP_FP64 && c->precision != DRP_FP32_SHIFTED)
+        return drp_fail(DRP_ERR_ARG, "unknown precision %d", c->precision);
+    if (!(c->tau_f > 0.5)) return drp_fail(DRP_ERR_ARG, "tau_f must exceed 1/2");
+    if (!(c->nu1 > 0.0) || !(c->nu2 > 0.0)) return drp_fail(DRP_ERR_ARG, "viscosities must be positive");
+    if (c->wall_axis != DRP_WALL_NONE && c->wall_axis != DRP_WALL_Y && c->wall_axis != DRP_WALL_Z)
+        return drp_fail(DRP_ERR_ARG, "wall_axis must be 0, 2 or 3");
+    if (c->force_kind < DRP_FORCE_NONE || c->force_kind > DRP_FORCE_FIELD)
+        return drp_fail(DRP_ERR_ARG, "unknown force_kind %d", c->force_kind);
+    if (c->anu != 0.0)
+        return drp_fail(DRP_ERR_UNSUPPORTED, "anu != 0 (velocity-gradient term of geq) is not implemented");
+    for (int k = 0; k < 8; k++)
+        if (c->reserved[k] != 0) return drp_fail(DRP_ERR_ARG, "reserved fields must be zero");
+    return DRP_OK;
+}
+
+static int pick_shape(const drp_config& c) {
+    if (const char* e = std::getenv("DROPLET_SHAPE")) {
+        int s = std::atoi(e);
+        if (s >= 0 && s < SHAPE_COUNT) return s;
+    }
+    (void)c;
+    return SHAPE_32x4x2;
+}
+
+extern "C" int drp_create(const drp_config* cfg, drp_lattice** out) {
+    if (!out) return drp_fail(DRP_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int rc = validate(cfg);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return drp_fail(DRP_ERR_HIP, "no HIP device");
+    drp_lattice* L = new (std::nothrow) drp_lattice();
+    if (!L) return drp_fail(DRP_ERR_ALLOC, "host allocation failed");
+    L->cfg = *cfg;
+    L->phys = make_phys(*cfg);
+    L->n = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+    L->shape = pick_shape(*cfg);
+    (void)hipGetDevice(&L->dev);
+    rc = lattice_alloc(L);
+    if (rc) {
+        lattice_free(L);
+        delete L;
+        return rc;
+    }
+    *out = L;
+    return DRP_OK;
+}
+
+extern "C" void drp_destroy(drp_lattice* L) {
+    if (!L) return;
+    lattice_free(L);
+    delete L;
+}
+
+int lattice_alloc(drp_lattice* L) {
+    const int64_t n = L->n;
+    auto dalloc = [&](void** p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes) != hipSuccess) {
+            *p = nullptr;
+            return drp_fail(DRP_ERR_ALLOC, "hipMalloc(%zu) failed", bytes);
+        }
+        return DRP_OK;
+    };
+    HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+    int rc = 0;
+    const size_t esz = (L->cfg.precision == DRP_FP64) ? sizeof(double) : sizeof(float);
+    for (int k = 0; k < 2 && !rc; k++) {
+        rc |= dalloc(&L->pop[k], 2 * 15 * (size_t)n * esz);
+        rc |= dalloc((void**)&L->mac[k], 5 * (size_t)n * esz);
+    }
+    if (!rc) rc = dalloc((void**)&L->range, 6 * sizeof(uint64_t));
+    if (!rc && L->phys.les) rc = dalloc((void**)&L->tau, (size_t)n * sizeof(double));
+    if (!rc && L->cfg.force_kind == DRP_FORCE_FIELD) {
+        rc = dalloc((void**)&L->force, 3 * (size_t)n * sizeof(double));
+        if (!rc) HIP_TRY(hipMemsetAsync(L->force, 0, 3 * (size_t)n * sizeof(double), L->stream));
+    }
+    if (!rc && L->cfg.force_kind == DRP_FORCE_TAYLOR_GREEN) {
+        const int nx = L->cfg.nx, nz = L->cfg.nz;
+        rc = dalloc((void**)&L->tg, (2 * (size_t)nx + 2 * (size_t)nz) * sizeof(double));
+        if (!rc) {
+            // sin/cos tables of kw*x and kw*z (externalforce :682-690), evaluated on the host
+            std::vector<double> t(2 * nx + 2 * nz);
+            const double k = L->cfg.force_k;
+            for (int x = 0; x < nx; x++) {
+                t[x] = std::sin(k * (double)x);
+                t[nx + x] = std::cos(k * (double)x);
+            }
+            for (int z = 0; z < nz; z++) {
+                t[2 * nx + z] = std::sin(k * (double)z);
+                t[2 * nx + nz + z] = std::cos(k * (double)z);
+            }
+            HIP_TRY(hipMemcpy(L->tg, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    return rc;
+}
+
+void lattice_free(drp_lattice* L) {
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    for (int k = 0; k < 2; k++) {
+        if (L->pop[k]) (void)hipFree(L->pop[k]);
+        if (L->mac[k]) (void)hipFree(L->mac[k]);
+    }
+    if (L->range) (void)hipFree(L->range);
+    if (L->tau) (void)hipFree(L->tau);
+    if (L->force) (void)hipFree(L->force);
+    if (L->tg) (void)hipFree(L->tg);
+    if (L->partial) (void)hipFree(L->partial);
+    if (L->stream) (void)hipStreamDestroy(L->stream);
+    L->stream = nullptr;
+}
+
+// -------------------------------------------------------------------- init
+
+// (re)start the rotating phi-range slots: slot 0 holds the range used by the
+// first step, (max, min) = (phi2, phi1) as ini_op uses (:278); slot 1 is in the
+// neutral state (max, min) = (phi1, phi2) that physics starts its scan from (:557-558).
+static int reset_range(drp_lattice* L) {
+    HIP_TRY(launch_range_set(L->range, 0, L->cfg.phi2, L->cfg.phi1, L->stream));
+    HIP_TRY(launch_range_set(L->range, 1, L->cfg.phi1, L->cfg.phi2, L->stream));
+    HIP_TRY(launch_range_set(L->range, 2, L->cfg.phi1, L->cfg.phi2, L->stream));
+    return DRP_OK;
+}
+
+static int init_from_host_fields(drp_lattice* L, const double* const* host) {
+    const int64_t n = L->n;
+    if (L->cfg.precision != DRP_FP64) return fp32_init_from_host_fields(L, host);
+    for (int k = 0; k < 5; k++)
+        HIP_TRY(hipMemcpyAsync((double*)L->mac[L->cur] + k * n, host[k], n * sizeof(double), hipMemcpyHostToDevice,
+                               L->stream));
+    double* f = (double*)L->pop[L->cur];
+    HIP_TRY(launch_init_equilibrium_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, L->phys, (double*)L->mac[L->cur], f,
+                                         f + 15 * n, L->stream));
+    int rc = reset_range(L);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    L->step = 0;
+    L->initialized = true;
+    return DRP_OK;
+}
+
+extern "C" int drp_init_droplets(drp_lattice* L, const drp_drop* drops, int ndrops, double p_init) {
+    if (!L) return drp_fail(DRP_ERR_ARG, "null lattice");
+    if (ndrops < 0 || (ndrops > 0 && !drops)) return drp_fail(DRP_ERR_ARG, "bad droplet list");
+    const drp_config& c = L->cfg;
+    const int64_t n = L->n;
+    std::vector<double> phi(n), p(n, p_init), zero(n, 0.0);
+    // sharp step, ini :224-235: phi2 where (x-xc)^2+(y-yc)^2+(z-zc)^2 <= (0.5 D)^2
+    for (int z = 0; z < c.nz; z++)
+        for (int y = 0; y < c.ny; y++)
+            for (int x = 0; x < c.nx; x++) {
+                double v = c.phi1;
+                for (int k = 0; k < ndrops; k++) {
+                    const double dx = (double)x - drops[k].xc, dy = (double)y - drops[k].yc,
+                                 dz = (double)z - drops[k].zc;
+                    const double dif = dx * dx + dy * dy + dz * dz;
+                    const double r = 0.5 * drops[k].diameter;
+                    if (dif <= r * r) v = c.phi2;
+                }
+                phi[x + (int64_t)c.nx * (y + (int64_t)c.ny * z)] = v;
+            }
+    const double* f[5] = {phi.data(), p.data(), zero.data(), zero.data(), zero.data()};
+    return init_from_host_fields(L, f);
+}
+
+extern "C" int drp_init_fields(drp_lattice* L, const double* phi, const double* p, const double* u1,
+                               const double* u2, const double* u3) {
+    if (!L || !phi || !p || !u1 || !u2 || !u3) return drp_fail(DRP_ERR_ARG, "null argument");
+    const double* f[5] = {phi, p, u1, u2, u3};
+    return init_from_host_fields(L, f);
+}
+
+// -------------------------------------------------------------------- step
+
+StepArgs make_step_args(drp_lattice* L, int64_t nstep) {
+    const drp_config& c = L->cfg;
+    const Phys& P = L->phys;
+    const int64_t n = L->n;
+    StepArgs a{};
+    a.nx = c.nx;
+    a.ny = c.ny;
+    a.nz = c.nz;
+    a.n = n;
+    double* fi = (double*)L->pop[L->cur];
+    double* fo = (double*)L->pop[1 - L->cur];
+    a.f_in = fi;
+    a.g_in = fi + 15 * n;
+    a.f_out = fo;
+    a.g_out = fo + 15 * n;
+    a.mac_in = (const double*)L->mac[L->cur];
+    a.mac_out = (double*)L->mac[1 - L->cur];
+    a.tau_field = P.les ? L->tau : nullptr;
+    a.tau_f = P.tau_f;
+    a.kf = P.kf;
+    a.kf6 = P.kf6;
+    a.kg = P.kg;
+    a.T = P.T;
+    a.a = P.a;
+    a.b = P.b;
+    a.nu1 = P.nu1;
+    a.dnu = P.dnu;
+    a.tg_amp = P.tg_amp;
+    a.phi1 = P.phi1;
+    a.phi2 = P.phi2;
+    a.wall_axis = c.wall_axis;
+    a.wall_u_now = (nstep >= c.wall_start) ? c.wall_u : 0.0; // bounce_back_LBM :670-674
+    a.force_kind = FORCE_NONE;
+    if (c.force_kind == DRP_FORCE_TAYLOR_GREEN && nstep >= c.force_start) {
+        a.force_kind = FORCE_TAYLOR_GREEN;
+        a.sinx = L->tg;
+        a.cosx = L->tg + c.nx;
+        a.sinz = L->tg + 2 * c.nx;
+        a.cosz = L->tg + 2 * c.nx + c.nz;
+    } else if (c.force_kind == DRP_FORCE_FIELD && nstep >= c.force_start) {
+        a.force_kind = FORCE_FIELD;
+        a.fx = L->force;
+        a.fy = L->force + n;
+        a.fz = L->force + 2 * n;
+    }
+    a.range = L->range;
+    a.slot_read = (int)((nstep - 1) % 3);
+    a.slot_write = (int)(nstep % 3);
+    a.slot_reset = (int)((nstep + 1) % 3);
+    return a;
+}
+
+extern "C" int drp_step(drp_lattice* L, int nsteps) {
+    if (!L) return drp_fail(DRP_ERR_ARG, "null lattice");
+    if (!L->initialized) return drp_fail(DRP_ERR_STATE, "drp_step before drp_init_*");
+    if (nsteps < 0) return drp_fail(DRP_ERR_ARG, "nsteps < 0");
+    if (L->cfg.precision != DRP_FP64) return fp32_step(L, nsteps);
+    for (int k = 0; k < nsteps; k++) {
+        const int64_t nstep = L->step + 1;
+        if (L->phys.les)
+            HIP_TRY(launch_les_tau_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, L->phys, (const double*)L->mac[L->cur],
+                                        L->range, (int)((nstep - 1) % 3), L->tau, L->stream));
+        StepArgs a = make_step_args(L, nstep);
+        HIP_TRY(launch_step_fp64(a, L->shape, L->stream));
+        L->cur = 1 - L->cur;
+        L->step = nstep;
+    }
+    return DRP_OK;
+}
+
+extern "C" int drp_sync(drp_lattice* L) {
+    if (!L) return drp_fail(DRP_ERR_ARG, "null lattice");
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return DRP_OK;
+}
+
+extern "C" int64_t drp_step_count(const drp_lattice* L) { return L ? L->step : -1; }
+
+// -------------------------------------------------------------------- access
+
+extern "C" int drp_get_macro(drp_lattice* L, int field, double* host, size_t n) {
+    if (!L || !host) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (field < 0 || field > 4) return drp_fail(DRP_ERR_ARG, "unknown field %d", field);
+    if (n != (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != %lld nodes", n, (long long)L->n);
+    if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
+    if (L->cfg.precision != DRP_FP64) return fp32_get_macro(L, field, host);
+    HIP_TRY(hipMemcpyAsync(host, (const double*)L->mac[L->cur] + field * L->n, n * sizeof(double),
+                           hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return DRP_OK;
+}
+
+extern "C" int drp_get_populations(drp_lattice* L, int set, double* host, size_t n) {
+    if (!L || !host) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (set != DRP_POP_PHI && set != DRP_POP_MOM) return drp_fail(DRP_ERR_ARG, "unknown population set %d", set);
+    if (n != 15 * (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != 15 x %lld", n, (long long)L->n);
+    if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
+    if (L->cfg.precision != DRP_FP64) return fp32_get_populations(L, set, host);
+    const double* src = (const double*)L->pop[L->cur] + (set == DRP_POP_MOM ? 15 * L->n : 0);
+    HIP_TRY(hipMemcpyAsync(host, src, n * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return DRP_OK;
+}
+
+static int set_pops(drp_lattice* L, int set, const double* host, size_t n, int which) {
+    if (!L || !host) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (set != DRP_POP_PHI && set != DRP_POP_MOM) return drp_fail(DRP_ERR_ARG, "unknown population set %d", set);
+    if (n != 15 * (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != 15 x %lld", n, (long long)L->n);
+    if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
+    if (L->cfg.precision != DRP_FP64) return fp32_set_populations(L, set, host, which);
+    double* base = (double*)L->pop[L->cur];
+    double* dst = base + (set == DRP_POP_MOM ? 15 * L->n : 0);
+    HIP_TRY(hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(launch_moments_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, base, base + 15 * L->n, (double*)L->mac[L->cur],
+                                which, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return DRP_OK;
+}
+
+extern "C" int drp_set_populations(drp_lattice* L, int set, const double* host, size_t n) {
+    return set_pops(L, set, host, n, set == DRP_POP_PHI ? 1 : 2);
+}
+
+extern "C" int drp_inject_momentum(drp_lattice* L, const double* g_host, size_t n) {
+    return set_pops(L, DRP_POP_MOM, g_host, n, 2);
+}
+
+extern "C" int drp_set_force(drp_lattice* L, const double* fx, const double* fy, const double* fz, size_t n) {
+    if (!L || !fx || !fy || !fz) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (L->cfg.force_kind != DRP_FORCE_FIELD) return drp_fail(DRP_ERR_STATE, "force_kind is not DRP_FORCE_FIELD");
+    if (n != (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != %lld nodes", n, (long long)L->n);
+    const double* src[3] = {fx, fy, fz};
+    for (int k = 0; k < 3; k++)
+        HIP_TRY(hipMemcpyAsync(L->force + k * L->n, src[k], n * sizeof(double), hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return DRP_OK;
+}
+
+// -------------------------------------------------------------------- diagnostics
+
+extern "C" int drp_reduce(drp_lattice* L, drp_stats* st) {
+    if (!L || !st) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
+    const double* mac = nullptr;
+    if (L->cfg.precision == DRP_FP64) {
+        mac = (const double*)L->mac[L->cur];
+    } else {
+        int rc = fp32_macro_fp64_view(L, &mac);
+        if (rc) return rc;
+    }
+    const int64_t nb = (L->n + 255) / 256;
+    if (L->partial_cap < nb) {
+        if (L->partial) (void)hipFree(L->partial);
+        L->partial = nullptr;
+        HIP_TRY(hipMalloc((void**)&L->partial, nb * NPART * sizeof(double)));
+        L->partial_cap = nb;
+    }
+    int nblocks = 0;
+    HIP_TRY(launch_reduce_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, L->phys, mac, L->partial, &nblocks, L->stream));
+    std::vector<double> part((size_t)nblocks * NPART);
+    uint64_t rg[2];
+    HIP_TRY(hipMemcpyAsync(part.data(), L->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipMemcpyAsync(rg, L->range + 2 * (L->step % 3), 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    double s[NPART] = {0};
+    for (int b = 0; b < nblocks; b++)
+        for (int k = 0; k < NPART; k++) s[k] += part[(size_t)b * NPART + k];
+    const double nn = (double)L->n;
+    std::memset(st, 0, sizeof(*st));
+    st->phi_mass = s[0];
+    st->phi_max = from_ordered_bits(rg[0]);
+    st->phi_min = from_ordered_bits(rg[1]);
+    st->kinetic = s[1] / nn;
+    st->dissipation = s[2] / nn * 0.5 * L->cfg.nu1;
+    st->interface_area = s[3];
+    st->free_energy = s[4];
+    st->p_mean = s[5] / nn;
+    st->momentum[0] = s[6];
+    st->momentum[1] = s[7];
+    st->momentum[2] = s[8];
+    return DRP_OK;
+}
+
+extern "C" int drp_deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do) {
+    if (!L || !Lo || !Bo || !Do) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
+    if (mode != 0 && mode != 1) return drp_fail(DRP_ERR_ARG, "unknown deformation mode %d", mode);
+    return deformation(L, mode, Lo, Bo, Do);
+}

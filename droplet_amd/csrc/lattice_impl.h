@@ -1,0 +1,49 @@
+// lattice_impl.h — internal state of a drp_lattice handle (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+
+#include "../../include/droplet.h"
+#include "kernels.h"
+
+struct drp_lattice {
+    drp_config cfg{};
+    drp::Phys phys{};
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0;
+    int64_t step = 0;      // reference loop counter of the last completed step
+    bool initialized = false;
+    int shape = 0;         // tile shape of the step kernel
+    int cur = 0;           // which ping-pong buffer holds the current state
+    void* pop[2] = {nullptr, nullptr};  // FP64: [f(15n) | g(15n)] doubles; FP32: floats
+    void* mac[2] = {nullptr, nullptr};  // phi, p, u1, u2, u3 (5n)
+    uint64_t* range = nullptr;          // 3 rotating (max, min) slots of the phi range
+    double* tau = nullptr;              // LES relaxation-time field
+    double* force = nullptr;            // FORCE_FIELD: fx | fy | fz
+    double* tg = nullptr;               // Taylor-Green sin/cos tables
+    double* partial = nullptr;          // diagnostics partial sums
+    int64_t partial_cap = 0;
+    double* mac64 = nullptr;            // FP32 mode: FP64 copy of the macroscopic fields
+    void* fp32_state = nullptr;         // FP32 mode: shift constants etc.
+};
+
+int drp_fail(int code, const char* fmt, ...);
+drp::Phys make_phys(const drp_config& c);
+int lattice_alloc(drp_lattice* L);
+void lattice_free(drp_lattice* L);
+drp::StepArgs make_step_args(drp_lattice* L, int64_t nstep);
+
+// deformation.cpp
+int deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do);
+int copy_phi_plane(drp_lattice* L, int third, double* host, int* n1, int* n2);
+
+// fp32_shifted.cpp
+int fp32_init_from_host_fields(drp_lattice* L, const double* const* host);
+int fp32_step(drp_lattice* L, int nsteps);
+int fp32_get_macro(drp_lattice* L, int field, double* host);
+int fp32_get_populations(drp_lattice* L, int set, double* host);
+int fp32_set_populations(drp_lattice* L, int set, const double* host, int which);
+int fp32_macro_fp64_view(drp_lattice* L, const double** mac);

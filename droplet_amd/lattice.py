@@ -1,0 +1,161 @@
+"""Python front-end of the HIP library (ctypes over include/droplet.h).
+
+Mirrors the reference driver's call surface: construct (par + allocation),
+init_droplets / init_fields (ini + ini_op), step (loop body), fields /
+populations (output / outputfg), inject_momentum (step_input injection),
+reduce and deformation (analysis hooks).
+
+The library must be built first (``python -m droplet_amd.build``); importing
+this module on a machine without it raises instead of falling back to any CPU
+path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import Config, Drop, Stats, POP_MOM, POP_PHI, FIELDS  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libdroplet_amd.so")
+_lib = None
+
+
+class DropletError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        msg = lib().drp_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libdroplet_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built; run `python -m droplet_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, sz = C.c_void_p, C.POINTER(C.c_double), C.c_size_t
+        L.drp_last_error.restype = C.c_char_p
+        L.drp_config_default.argtypes = [C.POINTER(Config)]
+        L.drp_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.drp_destroy.argtypes = [vp]
+        L.drp_destroy.restype = None
+        L.drp_init_droplets.argtypes = [vp, C.POINTER(Drop), C.c_int, C.c_double]
+        L.drp_init_fields.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.drp_step.argtypes = [vp, C.c_int]
+        L.drp_sync.argtypes = [vp]
+        L.drp_step_count.argtypes = [vp]
+        L.drp_step_count.restype = C.c_int64
+        L.drp_get_macro.argtypes = [vp, C.c_int, dp, sz]
+        L.drp_get_populations.argtypes = [vp, C.c_int, dp, sz]
+        L.drp_set_populations.argtypes = [vp, C.c_int, dp, sz]
+        L.drp_inject_momentum.argtypes = [vp, dp, sz]
+        L.drp_set_force.argtypes = [vp, dp, dp, dp, sz]
+        L.drp_reduce.argtypes = [vp, C.POINTER(Stats)]
+        L.drp_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc: int, where: str):
+    if rc != 0:
+        raise DropletError(rc, where)
+
+
+class Lattice:
+    """One D3Q15 two-phase lattice resident on the GPU."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        self.shape = (cfg.nz, cfg.ny, cfg.nx)
+        self.n = cfg.nx * cfg.ny * cfg.nz
+        h = C.c_void_p()
+        _check(lib().drp_create(C.byref(cfg), C.byref(h)), "drp_create")
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().drp_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- init (ini + ini_op)
+    def init_droplets(self, drops, p_init: float = 1.0 / 3.0):
+        arr = (Drop * max(1, len(drops)))(*[Drop(*d) for d in drops])
+        _check(lib().drp_init_droplets(self._h, arr, len(drops), p_init), "drp_init_droplets")
+
+    def init_fields(self, phi, p, u1, u2, u3):
+        fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (phi, p, u1, u2, u3)]
+        _check(lib().drp_init_fields(self._h, *[_dp(a) for a in fs]), "drp_init_fields")
+
+    # -- time stepping
+    def step(self, n: int = 1):
+        _check(lib().drp_step(self._h, int(n)), "drp_step")
+
+    def sync(self):
+        _check(lib().drp_sync(self._h), "drp_sync")
+
+    @property
+    def step_count(self) -> int:
+        return int(lib().drp_step_count(self._h))
+
+    # -- output hooks
+    def macro(self, field: int) -> np.ndarray:
+        out = np.empty(self.shape, np.float64)
+        _check(lib().drp_get_macro(self._h, int(field), _dp(out), out.size), "drp_get_macro")
+        return out
+
+    def fields(self) -> dict:
+        return {k: self.macro(i) for k, i in FIELDS.items()}
+
+    def populations(self, which: int) -> np.ndarray:
+        out = np.empty((15,) + self.shape, np.float64)
+        _check(lib().drp_get_populations(self._h, int(which), _dp(out), out.size), "drp_get_populations")
+        return out
+
+    def set_populations(self, which: int, pops: np.ndarray):
+        a = np.ascontiguousarray(pops, np.float64)
+        if a.shape != (15,) + self.shape:
+            raise ValueError(f"populations must have shape {(15,) + self.shape}")
+        _check(lib().drp_set_populations(self._h, int(which), _dp(a), a.size), "drp_set_populations")
+
+    def inject_momentum(self, g: np.ndarray):
+        a = np.ascontiguousarray(g, np.float64)
+        if a.shape != (15,) + self.shape:
+            raise ValueError(f"populations must have shape {(15,) + self.shape}")
+        _check(lib().drp_inject_momentum(self._h, _dp(a), a.size), "drp_inject_momentum")
+
+    def set_force(self, fx, fy, fz):
+        fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (fx, fy, fz)]
+        _check(lib().drp_set_force(self._h, *[_dp(a) for a in fs], self.n), "drp_set_force")
+
+    # -- analysis hooks
+    def reduce(self) -> dict:
+        st = Stats()
+        _check(lib().drp_reduce(self._h, C.byref(st)), "drp_reduce")
+        return st.as_dict()
+
+    def deformation(self, mode: int = 0):
+        L, B, D = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().drp_deformation(self._h, int(mode), C.byref(L), C.byref(B), C.byref(D)), "drp_deformation")
+        return L.value, B.value, D.value

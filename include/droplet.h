@@ -1,0 +1,168 @@
+/*
+ * droplet.h — C ABI of the MI355X-native D3Q15 two-phase lattice Boltzmann step.
+ *
+ * The reference (nakano-gendai/droplet) has no library boundary: its "API" is the
+ * fixed call order of `program main` over `module globals`,
+ *     par → ini → ini_op → DO n: [step_input injection + externalforce]
+ *       → MPI_boundary → various_cal → gphi_cal → equilibrium_cal → MPI_boundary_fg
+ *       → fg_cal → [bounce_back_LBM] → physics → [output | outputfg | break_up3]
+ * (program/lbm_drop_xy.f90:774-868, prototype_program/two-phase/dropch.f90:1079-1912).
+ * Each entry point below replaces one group of those calls; the comment on each
+ * names the reference routine(s) it stands for.  The Fortran driver binds these
+ * through iso_c_binding (droplet_amd/fortran/droplet_amd.f90), Python through ctypes
+ * (droplet_amd/lattice.py).
+ *
+ * Conventions
+ *  - Return 0 on success, a negative DRP_ERR_* code on failure; never abort.
+ *    drp_last_error() gives a message for the calling thread's last failure.
+ *  - The library owns all device memory; callers own host buffers.
+ *  - One lattice handle is bound to one HIP stream; calls on one handle must be
+ *    serialised by the caller.  Separate handles are independent.
+ *  - Naming: "phi populations" are the order-parameter populations (reference `f`),
+ *    "momentum populations" carry pressure and momentum (reference `g`).
+ *  - Host field layout is x fastest, then y, then z: index = x + nx*(y + ny*z).
+ *    Population arrays are [q][z][y][x] with q = 0..14 ↔ reference i = 1..15.
+ *  - Lattice spacing ds = 1 (every reference file uses ds = 1.0d0).
+ */
+#ifndef DROPLET_AMD_H
+#define DROPLET_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRP_OK 0
+#define DRP_ERR_ARG (-1)         /* invalid argument / size mismatch */
+#define DRP_ERR_HIP (-2)         /* HIP runtime failure              */
+#define DRP_ERR_STATE (-3)       /* call out of order (e.g. step before init) */
+#define DRP_ERR_UNSUPPORTED (-4) /* configuration not implemented on this path */
+#define DRP_ERR_ALLOC (-5)       /* device allocation failed         */
+
+/* storage/arithmetic precision of the populations */
+#define DRP_FP64 0          /* FP64 storage and arithmetic (reference precision) */
+#define DRP_FP32_SHIFTED 1  /* FP32 deviations from a rest equilibrium, FP32 collision */
+
+/* walls: 0 = fully periodic; 2 = walls normal to y; 3 = walls normal to z (reference dropch) */
+#define DRP_WALL_NONE 0
+#define DRP_WALL_Y 2
+#define DRP_WALL_Z 3
+
+/* body force kinds */
+#define DRP_FORCE_NONE 0
+#define DRP_FORCE_TAYLOR_GREEN 1 /* externalforce, program/lbm_drop_xy.f90:668-693 */
+#define DRP_FORCE_FIELD 2        /* user-supplied static field (drp_set_force)      */
+
+/* macroscopic fields for drp_get_macro / drp_set_macro */
+#define DRP_FIELD_PHI 0
+#define DRP_FIELD_P 1
+#define DRP_FIELD_U1 2
+#define DRP_FIELD_U2 3
+#define DRP_FIELD_U3 4
+
+/* population sets */
+#define DRP_POP_PHI 0 /* reference f */
+#define DRP_POP_MOM 1 /* reference g */
+
+typedef struct drp_config {
+    int32_t nx, ny, nz;   /* lattice nodes per axis (reference xmax+1, ymax+1, zmax+1) */
+    int32_t precision;    /* DRP_FP64 | DRP_FP32_SHIFTED                              */
+    double tau_f;         /* phi-population relaxation time (ref tauf)                 */
+    double nu1, nu2;      /* kinematic viscosity of continuous / dispersed phase       */
+    double kappa_f;       /* interface-width parameter (ref kappaf)                    */
+    double kappa_g;       /* surface-tension parameter (ref kappag)                    */
+    double phi1, phi2;    /* order parameter of continuous / dispersed phase           */
+    double a, b, T;       /* van der Waals constants of p0 = phi T/(1-b phi) - a phi^2 */
+    double anu;           /* ref Anu (velocity-gradient term in geq); 0 in every run   */
+    int32_t wall_axis;    /* DRP_WALL_*                                                */
+    int32_t wall_start;   /* walls move (+U top, -U bottom, along x) from this step (ref start) */
+    double wall_u;        /* wall speed U (ref uw)                                     */
+    int32_t force_kind;   /* DRP_FORCE_*                                               */
+    int32_t force_start;  /* first step whose collision includes the force (ref step_input) */
+    double force_amp;     /* Taylor-Green amplitude 2 pi^2 U^2/(D_vortex Re)           */
+    double force_k;       /* Taylor-Green wavenumber pi/D_vortex (ref kw)              */
+    double les_cs;        /* Smagorinsky constant Cs (0 = no LES)                      */
+    double les_delta;     /* filter width Delta                                        */
+    int32_t reserved[8];  /* must be zero                                              */
+} drp_config;
+
+typedef struct drp_drop {
+    double xc, yc, zc;    /* centre in lattice units                                   */
+    double diameter;      /* sharp step: phi2 where |x-c|^2 <= (D/2)^2 (ref ini :224-242) */
+} drp_drop;
+
+typedef struct drp_stats {
+    double phi_mass;      /* sum phi                                                   */
+    double phi_min, phi_max; /* range used by nu(phi) at the next step (ref physics :559-574) */
+    double kinetic;       /* 0.5 <|u|^2>  (mean over nodes)                            */
+    double dissipation;   /* nu1 <2 S:S> (mean over nodes)                             */
+    double interface_area;/* sum |grad phi| (ref drop_analysis.f90:445-474)            */
+    double free_energy;   /* sum psi(phi) + kappa_f/2 |grad phi|^2                     */
+    double p_mean;        /* <p>                                                       */
+    double momentum[3];   /* sum u                                                     */
+} drp_stats;
+
+typedef struct drp_lattice drp_lattice; /* opaque; owns all device memory */
+
+/* Fill a config with the reference defaults of program/lbm_drop_xy.f90:9-67
+ * (phi1=2.211, phi2=4.895, a=9/49, b=2/21, T=0.55, kappa_f=0.01, tau_f=0.7). */
+int drp_config_default(drp_config* cfg);
+
+/* = par + allocation part of ini (program/lbm_drop_xy.f90:117-129, 131-222). */
+int drp_create(const drp_config* cfg, drp_lattice** out);
+
+/* = ini + ini_op with the reference sharp-step droplet(s): phi, u=0, p=p_init,
+ * then f := feq, g := geq and the nu(phi) range (phi1, phi2)
+ * (program/lbm_drop_xy.f90:224-284; prototype_program/two-phase/dropch.f90:186-473). */
+int drp_init_droplets(drp_lattice* lat, const drp_drop* drops, int ndrops, double p_init);
+
+/* = ini from arbitrary host fields (each nx*ny*nz doubles): populations := equilibria. */
+int drp_init_fields(drp_lattice* lat, const double* phi, const double* p,
+                    const double* u1, const double* u2, const double* u3);
+
+/* Advance nsteps time steps (= loop body program/lbm_drop_xy.f90:787-848 with
+ * bounce_back_LBM of dropch.f90:1097 when walls are configured).  Asynchronous
+ * on the handle's stream. */
+int drp_step(drp_lattice* lat, int nsteps);
+
+/* Wait for all work queued on the handle. */
+int drp_sync(drp_lattice* lat);
+
+/* Number of steps taken since init (the reference loop counter n). */
+int64_t drp_step_count(const drp_lattice* lat);
+
+/* = output (program/lbm_drop_xy.f90:695-719): copy one macroscopic field to host (FP64). */
+int drp_get_macro(drp_lattice* lat, int field, double* host, size_t n);
+
+/* = outputfg (program/lbm_drop_xy.f90:721-743): populations to host, [q][z][y][x], FP64. */
+int drp_get_populations(drp_lattice* lat, int set, double* host, size_t n);
+
+/* Restart: set one population set from host and recompute the moments it carries. */
+int drp_set_populations(drp_lattice* lat, int set, const double* host, size_t n);
+
+/* = step_input injection (program/lbm_drop_xy.f90:789-822): replace the momentum
+ * populations, recompute p and u from them; phi and the phi populations are untouched. */
+int drp_inject_momentum(drp_lattice* lat, const double* g_host, size_t n);
+
+/* Static body-force field for DRP_FORCE_FIELD (three nx*ny*nz arrays). */
+int drp_set_force(drp_lattice* lat, const double* fx, const double* fy, const double* fz, size_t n);
+
+/* Diagnostics reductions (wavefront -> workgroup -> grid, FP64 accumulation). */
+int drp_reduce(drp_lattice* lat, drp_stats* stats);
+
+/* Taylor deformation D = (L-B)/(L+B) in the shear plane.
+ * mode 0: break_up3 (prototype_program/two-phase/dropch.f90:857-948);
+ * mode 1: L = 2 max r, B = 4 area/(pi L) (prototype_program/two-phase/drop.f90:439-463). */
+int drp_deformation(drp_lattice* lat, int mode, double* L, double* B, double* D);
+
+/* Message for the calling thread's last failure ("" if none). */
+const char* drp_last_error(void);
+
+void drp_destroy(drp_lattice* lat);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DROPLET_AMD_H */

@@ -1,0 +1,614 @@
+/*
+ * lbm_oracle.c — CPU restatement of the reference D3Q15 two-phase LBM step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: tests/, the smoke check
+ * and the CPU-baseline leg of the benchmark load it; the product (the HIP
+ * library under droplet_amd/csrc) never links or calls it.
+ *
+ * It restates, in plain C (FP64, no FMA contraction), the algorithm of
+ *   program/lbm_drop_xy.f90            (periodic step, body force)
+ *   prototype_program/two-phase/dropch.f90 (moving-wall bounce-back, break_up3)
+ *   program/LES_drop_xy.f90            (Smagorinsky eddy viscosity)
+ * Every arithmetic expression keeps the reference's evaluation order (Fortran
+ * evaluates a chain of + or * left to right), so that on the same inputs the
+ * restatement and the compiled reference round identically.
+ *
+ * Layout: SoA, population q (0..14 = reference i 1..15) slowest, then z, y, x;
+ * periodic index wrap replaces the reference's 1-cell halos (glue/glue_f,
+ * program/lbm_drop_xy.f90:287-384).
+ *
+ * Parity pins (tests/test_oracle_*.py): golden vectors produced by the compiled
+ * reference (tests/golden/), lattice identities, moment identities, polynomial
+ * stencils, mass conservation, Couette flow between moving walls, Taylor-Green
+ * decay, analytic-ellipse deformation and closed-form diagnostics.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/droplet.h"
+
+/* D3Q15 table, program/lbm_drop_xy.f90:70-84 */
+static const int CX[15] = {0, 1, 0, 0, -1, 0, 0, 1, -1, 1, 1, -1, 1, -1, -1};
+static const int CY[15] = {0, 0, 1, 0, 0, -1, 0, 1, 1, -1, 1, -1, -1, 1, -1};
+static const int CZ[15] = {0, 0, 0, 1, 0, 0, -1, 1, 1, 1, -1, -1, -1, -1, 1};
+static const int OPP[15] = {0, 4, 5, 6, 1, 2, 3, 11, 12, 13, 14, 7, 8, 9, 10};
+#define E0 (2.0 / 9.0)
+#define E1 (1.0 / 9.0)
+#define E2 (1.0 / 72.0)
+static const double EW[15] = {E0, E1, E1, E1, E1, E1, E1, E2, E2, E2, E2, E2, E2, E2, E2};
+static const double HW[15] = {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+static const double FW[15] = {-7.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0,
+                              1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0,
+                              1.0 / 24.0, 1.0 / 24.0};
+
+typedef struct ora_lattice {
+    drp_config c;
+    int64_t n;        /* nodes */
+    int64_t step;     /* reference loop counter n of the last completed step */
+    double *f, *g;    /* populations [15][n] (reference f = phi pops, g = momentum pops) */
+    double *fn, *gn;  /* next-step buffers */
+    double *phi, *p, *u1, *u2, *u3;
+    double *fx, *fy, *fz; /* stored body force (zero when inactive) */
+    double *feq, *geq;    /* [15][n] */
+    double *lap, *gp;     /* lap phi, grad phi [3][n] */
+    double *gu;           /* grad u [9][n], gu[k*3+a] = d_a u_k (ref grad_u(k,a)) */
+    double *tau;          /* tau_g */
+    double phi_min, phi_max; /* range used by nu(phi) in the next step */
+    int threads;
+} ora_lattice;
+
+static inline int64_t idx(const ora_lattice* L, int x, int y, int z) {
+    return (int64_t)x + (int64_t)L->c.nx * ((int64_t)y + (int64_t)L->c.ny * z);
+}
+static inline int wrap(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+static inline int64_t nbr(const ora_lattice* L, int x, int y, int z, int i) {
+    return idx(L, wrap(x + CX[i], L->c.nx), wrap(y + CY[i], L->c.ny), wrap(z + CZ[i], L->c.nz));
+}
+static inline int64_t upstream(const ora_lattice* L, int x, int y, int z, int i) {
+    return idx(L, wrap(x - CX[i], L->c.nx), wrap(y - CY[i], L->c.ny), wrap(z - CZ[i], L->c.nz));
+}
+
+ora_lattice* ora_create(const drp_config* c) {
+    if (!c || c->nx < 3 || c->ny < 3 || c->nz < 3) return NULL;
+    ora_lattice* L = (ora_lattice*)calloc(1, sizeof(ora_lattice));
+    if (!L) return NULL;
+    L->c = *c;
+    L->n = (int64_t)c->nx * c->ny * c->nz;
+    int64_t n = L->n;
+    double** bufs[] = {&L->f, &L->g, &L->fn, &L->gn, &L->feq, &L->geq};
+    for (int k = 0; k < 6; k++) *bufs[k] = (double*)calloc((size_t)(15 * n), sizeof(double));
+    double** flds[] = {&L->phi, &L->p, &L->u1, &L->u2, &L->u3, &L->fx, &L->fy, &L->fz, &L->lap, &L->tau};
+    for (int k = 0; k < 10; k++) *flds[k] = (double*)calloc((size_t)n, sizeof(double));
+    L->gp = (double*)calloc((size_t)(3 * n), sizeof(double));
+    L->gu = (double*)calloc((size_t)(9 * n), sizeof(double));
+    L->phi_min = c->phi1;
+    L->phi_max = c->phi2;
+    L->threads = 1;
+    return L;
+}
+
+void ora_destroy(ora_lattice* L) {
+    if (!L) return;
+    double* all[] = {L->f, L->g, L->fn, L->gn, L->feq, L->geq, L->phi, L->p, L->u1, L->u2, L->u3,
+                     L->fx, L->fy, L->fz, L->lap, L->tau, L->gp, L->gu};
+    for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); k++) free(all[k]);
+    free(L);
+}
+
+void ora_set_threads(ora_lattice* L, int t) { L->threads = t < 1 ? 1 : t; }
+int64_t ora_step_count(const ora_lattice* L) { return L->step; }
+
+/* p0, lap phi, grad phi, grad u at one node: various_cal, program/lbm_drop_xy.f90:590-640 */
+static void stencils_at(ora_lattice* L, int x, int y, int z) {
+    int64_t s = idx(L, x, y, z);
+    int64_t nb[15];
+    for (int i = 1; i < 15; i++) nb[i] = nbr(L, x, y, z, i);
+    const double* phi = L->phi;
+    double lap = -14.0 * phi[s];                    /* :608 */
+    for (int i = 1; i < 15; i++) lap = lap + phi[nb[i]];
+    L->lap[s] = lap / 5.0;                          /* :613, 5 ds^2 */
+    for (int a = 0; a < 3; a++) {                   /* :616-633 */
+        const int* ca = a == 0 ? CX : (a == 1 ? CY : CZ);
+        double gphi = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+        for (int i = 1; i < 15; i++) {
+            double cr = (double)ca[i];
+            int64_t t = nb[i];
+            gphi = gphi + cr * phi[t];
+            g1 = g1 + cr * L->u1[t];
+            g2 = g2 + cr * L->u2[t];
+            g3 = g3 + cr * L->u3[t];
+        }
+        L->gp[a * L->n + s] = gphi / 10.0;          /* 10 ds */
+        L->gu[(0 * 3 + a) * L->n + s] = g1 / 10.0;
+        L->gu[(1 * 3 + a) * L->n + s] = g2 / 10.0;
+        L->gu[(2 * 3 + a) * L->n + s] = g3 / 10.0;
+    }
+}
+
+/* Smagorinsky eddy viscosity from grad u: program/LES_drop_xy.f90:661-675 (strain :481-501) */
+static double eddy_viscosity(const ora_lattice* L, int64_t s) {
+    if (L->c.les_cs == 0.0 && L->c.les_delta == 0.0) return 0.0;
+    double acc = 0.0;
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+            double sab = 0.5 * (L->gu[(a * 3 + b) * L->n + s] + L->gu[(b * 3 + a) * L->n + s]);
+            acc = acc + sab * sab;
+        }
+    double nue = sqrt(2.0 * acc);
+    return L->c.les_cs * L->c.les_cs * L->c.les_delta * L->c.les_delta * nue;
+}
+
+/* nu(phi), tau_g: physics, program/lbm_drop_xy.f90:576-587 (LES form LES_drop_xy.f90:682-693) */
+static double relaxation_time(const ora_lattice* L, int64_t s) {
+    double range = L->phi_max - L->phi_min;
+    double frac = (L->phi[s] - L->phi_min) / range;
+    /* Guard: the reference yields 0/0 = NaN when phi_max == phi_min; here the
+       fraction is taken as 0 in that case (identical results whenever nu1 == nu2). */
+    if (range == 0.0) frac = 0.0;
+    double nu = frac * (L->c.nu2 - L->c.nu1) + L->c.nu1;
+    double nue = eddy_viscosity(L, s);
+    return 0.5 + 3.0 * (nu + nue) + 2.0 / 3.0 * L->c.anu;
+}
+
+/* Korteweg tensor G_ab = 4.5 d_a phi d_b phi - 1.5 |grad phi|^2 delta_ab: gphi_cal :642-666 */
+static void korteweg_tensor(const double g[3], double G[3][3]) {
+    double s2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) G[a][b] = 4.5 * g[a] * g[b] - 1.5 * s2 * (a == b ? 1.0 : 0.0);
+}
+
+/* G_ab c_a c_b, summed beta-outer alpha-inner as equilibrium_cal :439-449 */
+static double korteweg_contraction(double G[3][3], int i) {
+    const double cr[3] = {(double)CX[i], (double)CY[i], (double)CZ[i]};
+    double acc = 0.0;
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) acc = acc + G[a][b] * cr[a] * cr[b];
+    return acc;
+}
+
+static double velocity_gradient_contraction(const ora_lattice* L, int64_t s, int i) {
+    const double cr[3] = {(double)CX[i], (double)CY[i], (double)CZ[i]};
+    double acc = 0.0;
+    for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++)
+            acc = acc + (L->gu[(b * 3 + a) * L->n + s] + L->gu[(a * 3 + b) * L->n + s]) * cr[a] * cr[b];
+    return acc;
+}
+
+/* feq_i, geq_i at one node: equilibrium_cal, program/lbm_drop_xy.f90:421-476 */
+static void equilibria_at(ora_lattice* L, int64_t s) {
+    const drp_config* c = &L->c;
+    double phi = L->phi[s], p = L->p[s], u1 = L->u1[s], u2 = L->u2[s], u3 = L->u3[s];
+    double g[3] = {L->gp[s], L->gp[L->n + s], L->gp[2 * L->n + s]};
+    double p0 = phi * c->T / (1.0 - c->b * phi) - c->a * (phi * phi);         /* :606 */
+    double bracket = p0 - c->kappa_f * phi * L->lap[s]
+                     - c->kappa_f / 6.0 * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]); /* :453-455 */
+    double usq = u1 * u1 + u2 * u2 + u3 * u3;
+    double G[3][3];
+    korteweg_tensor(g, G);
+    for (int i = 0; i < 15; i++) {
+        double gt = korteweg_contraction(G, i);
+        double ft = (c->anu != 0.0) ? velocity_gradient_contraction(L, s, i) : 0.0;
+        double cu_f = (double)CX[i] * u1 + (double)CY[i] * u2 + (double)CZ[i] * u3;
+        L->feq[i * L->n + s] = HW[i] * phi + FW[i] * bracket + EW[i] * phi * 3.0 * cu_f
+                               + EW[i] * c->kappa_f * gt;                           /* :452-459 */
+        double cu = u1 * (double)CX[i] + u2 * (double)CY[i] + u3 * (double)CZ[i];
+        L->geq[i * L->n + s] = EW[i] * (3.0 * p + 3.0 * cu + 4.5 * (cu * cu) - 1.5 * usq + c->anu * ft)
+                               + EW[i] * c->kappa_g * gt;                           /* :461-469 */
+    }
+}
+
+static void compute_equilibria(ora_lattice* L) {
+    const int nx = L->c.nx, ny = L->c.ny, nz = L->c.nz;
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) stencils_at(L, x, y, z);
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+    for (int64_t s = 0; s < L->n; s++) {
+        equilibria_at(L, s);
+        L->tau[s] = relaxation_time(L, s);
+    }
+}
+
+/* moments phi = sum f, p = sum g/3, u = sum c g: physics, program/lbm_drop_xy.f90:541-553 */
+static void moments_at(ora_lattice* L, int64_t s, int which) {
+    double phi = 0.0, p = 0.0, u1 = 0.0, u2 = 0.0, u3 = 0.0;
+    for (int i = 0; i < 15; i++) {
+        double gi = L->g[i * L->n + s];
+        phi = phi + L->f[i * L->n + s];
+        p = p + gi / 3.0;
+        u1 = u1 + (double)CX[i] * gi;
+        u2 = u2 + (double)CY[i] * gi;
+        u3 = u3 + (double)CZ[i] * gi;
+    }
+    if (which & 1) L->phi[s] = phi;
+    if (which & 2) {
+        L->p[s] = p;
+        L->u1[s] = u1;
+        L->u2[s] = u2;
+        L->u3[s] = u3;
+    }
+}
+
+/* global phi range: physics :557-574.  The reference scans with `if (>max) ...
+   elseif (<min)` from max=phi1, min=phi2 per MPI rank; the true extremes clamped
+   by the same initial values are used here. */
+static void phi_range(ora_lattice* L) {
+    double mx = L->c.phi1, mn = L->c.phi2;
+    for (int64_t s = 0; s < L->n; s++) {
+        double v = L->phi[s];
+        if (v > mx) mx = v;
+        if (v < mn) mn = v;
+    }
+    L->phi_max = mx;
+    L->phi_min = mn;
+}
+
+/* Taylor-Green / Kolmogorov body force: externalforce, program/lbm_drop_xy.f90:668-693 */
+static void taylor_green_force(ora_lattice* L) {
+    const drp_config* c = &L->c;
+    for (int z = 0; z < c->nz; z++)
+        for (int y = 0; y < c->ny; y++)
+            for (int x = 0; x < c->nx; x++) {
+                int64_t s = idx(L, x, y, z);
+                L->fx[s] = c->force_amp * sin(c->force_k * (double)x) * cos(c->force_k * (double)z);
+                L->fy[s] = 0.0;
+                L->fz[s] = -(c->force_amp * cos(c->force_k * (double)x) * sin(c->force_k * (double)z));
+            }
+}
+
+/* ---------------------------------------------------------------- initialisation */
+
+static void init_equilibrium_populations(ora_lattice* L) {
+    /* ini: f := feq, g := geq (program/lbm_drop_xy.f90:244-251) */
+    L->phi_min = L->c.phi1; /* ini_op uses (phi1, phi2) as the range, :273-284 */
+    L->phi_max = L->c.phi2;
+    compute_equilibria(L);
+    memcpy(L->f, L->feq, sizeof(double) * 15 * L->n);
+    memcpy(L->g, L->geq, sizeof(double) * 15 * L->n);
+    L->step = 0;
+}
+
+int ora_init_droplets(ora_lattice* L, const drp_drop* d, int nd, double p_init) {
+    const drp_config* c = &L->c;
+    for (int z = 0; z < c->nz; z++)
+        for (int y = 0; y < c->ny; y++)
+            for (int x = 0; x < c->nx; x++) {
+                int64_t s = idx(L, x, y, z);
+                double v = c->phi1;
+                for (int k = 0; k < nd; k++) {   /* :224-235 */
+                    double dx = (double)x - d[k].xc, dy = (double)y - d[k].yc, dz = (double)z - d[k].zc;
+                    double dif = dx * dx + dy * dy + dz * dz;
+                    double r = 0.5 * d[k].diameter;
+                    if (dif <= r * r) v = c->phi2;
+                }
+                L->phi[s] = v;
+                L->p[s] = p_init;
+                L->u1[s] = L->u2[s] = L->u3[s] = 0.0;
+            }
+    init_equilibrium_populations(L);
+    return 0;
+}
+
+int ora_init_fields(ora_lattice* L, const double* phi, const double* p, const double* u1,
+                    const double* u2, const double* u3) {
+    size_t b = sizeof(double) * (size_t)L->n;
+    memcpy(L->phi, phi, b);
+    memcpy(L->p, p, b);
+    memcpy(L->u1, u1, b);
+    memcpy(L->u2, u2, b);
+    memcpy(L->u3, u3, b);
+    init_equilibrium_populations(L);
+    return 0;
+}
+
+/* ---------------------------------------------------------------- the step */
+
+static void stream_collide(ora_lattice* L, int force_on) {
+    const drp_config* c = &L->c;
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const int64_t n = L->n;
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                int64_t s = idx(L, x, y, z);
+                for (int i = 0; i < 15; i++) {
+                    int64_t u = upstream(L, x, y, z, i);
+                    double fs = L->f[i * n + u], gs = L->g[i * n + u];
+                    /* fg_cal, program/lbm_drop_xy.f90:497-508 */
+                    L->fn[i * n + s] = fs - (fs - L->feq[i * n + u]) / c->tau_f;
+                    double gv = gs - (gs - L->geq[i * n + u]) / L->tau[u];
+                    if (force_on)
+                        gv = gv + 3.0 * EW[i] * ((double)CX[i] * L->fx[u] + (double)CY[i] * L->fy[u]
+                                                 + (double)CZ[i] * L->fz[u]);
+                    L->gn[i * n + s] = gv;
+                }
+            }
+}
+
+/* moving-wall bounce-back on the momentum populations of the first/last fluid
+   layer: bounce_back_LBM, prototype_program/two-phase/dropch.f90:666-698 */
+static void bounce_back(ora_lattice* L) {
+    const drp_config* c = &L->c;
+    if (c->wall_axis != DRP_WALL_Y && c->wall_axis != DRP_WALL_Z) return;
+    double U = (L->step + 1 >= c->wall_start) ? c->wall_u : 0.0;  /* :670-674, n >= start */
+    const int n_axis = (c->wall_axis == DRP_WALL_Y) ? c->ny : c->nz;
+    for (int side = 0; side < 2; side++) {
+        int layer = side == 0 ? 0 : n_axis - 1;
+        double uw = side == 0 ? -U : U;  /* bottom wall moves with -U, top with +U */
+        for (int i = 0; i < 15; i++) {
+            int ca = (c->wall_axis == DRP_WALL_Y) ? CY[i] : CZ[i];
+            if (ca != (side == 0 ? 1 : -1)) continue; /* populations entering the fluid */
+            int o = OPP[i];
+            double w = 6.0 * EW[o];
+            double dot = uw * (double)CX[o] + 0.0 * (double)CY[o] + 0.0 * (double)CZ[o];
+            for (int a = 0; a < (c->wall_axis == DRP_WALL_Y ? c->nz : c->ny); a++)
+                for (int x = 0; x < c->nx; x++) {
+                    int64_t s = (c->wall_axis == DRP_WALL_Y) ? idx(L, x, layer, a) : idx(L, x, a, layer);
+                    L->gn[i * L->n + s] = L->gn[o * L->n + s] - w * dot;
+                }
+        }
+    }
+}
+
+int ora_step(ora_lattice* L, int nsteps) {
+    const drp_config* c = &L->c;
+    for (int k = 0; k < nsteps; k++) {
+        int64_t nstep = L->step + 1; /* reference loop counter n */
+        int force_on = 0;
+        if (c->force_kind == DRP_FORCE_TAYLOR_GREEN && nstep >= c->force_start) {
+            if (nstep == c->force_start) taylor_green_force(L);
+            force_on = 1;
+        } else if (c->force_kind == DRP_FORCE_FIELD && nstep >= c->force_start) {
+            force_on = 1;
+        }
+        compute_equilibria(L);      /* various_cal, gphi_cal, equilibrium_cal (+ tau of physics) */
+        stream_collide(L, force_on); /* fg_cal */
+        bounce_back(L);              /* bounce_back_LBM */
+        double* t;
+        t = L->f; L->f = L->fn; L->fn = t;
+        t = L->g; L->g = L->gn; L->gn = t;
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+        for (int64_t s = 0; s < L->n; s++) moments_at(L, s, 3);
+        phi_range(L);
+        L->step = nstep;
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- access */
+
+static double* field_ptr(ora_lattice* L, int f) {
+    switch (f) {
+        case DRP_FIELD_PHI: return L->phi;
+        case DRP_FIELD_P: return L->p;
+        case DRP_FIELD_U1: return L->u1;
+        case DRP_FIELD_U2: return L->u2;
+        case DRP_FIELD_U3: return L->u3;
+        default: return NULL;
+    }
+}
+
+int ora_get_macro(ora_lattice* L, int field, double* out) {
+    double* src = field_ptr(L, field);
+    if (!src) return DRP_ERR_ARG;
+    memcpy(out, src, sizeof(double) * (size_t)L->n);
+    return 0;
+}
+
+int ora_get_populations(ora_lattice* L, int set, double* out) {
+    memcpy(out, set == DRP_POP_PHI ? L->f : L->g, sizeof(double) * 15 * (size_t)L->n);
+    return 0;
+}
+
+int ora_set_populations(ora_lattice* L, int set, const double* in) {
+    memcpy(set == DRP_POP_PHI ? L->f : L->g, in, sizeof(double) * 15 * (size_t)L->n);
+    for (int64_t s = 0; s < L->n; s++) moments_at(L, s, set == DRP_POP_PHI ? 1 : 2);
+    return 0;
+}
+
+/* step_input injection, program/lbm_drop_xy.f90:789-822: g replaced, p and u
+   recomputed from it, phi and f untouched. */
+int ora_inject_momentum(ora_lattice* L, const double* g) {
+    memcpy(L->g, g, sizeof(double) * 15 * (size_t)L->n);
+    for (int64_t s = 0; s < L->n; s++) moments_at(L, s, 2);
+    return 0;
+}
+
+int ora_set_force(ora_lattice* L, const double* fx, const double* fy, const double* fz) {
+    size_t b = sizeof(double) * (size_t)L->n;
+    memcpy(L->fx, fx, b);
+    memcpy(L->fy, fy, b);
+    memcpy(L->fz, fz, b);
+    return 0;
+}
+
+void ora_phi_range(const ora_lattice* L, double* mn, double* mx) {
+    *mn = L->phi_min;
+    *mx = L->phi_max;
+}
+
+/* ---------------------------------------------------------------- diagnostics */
+
+/* Serial, fixed-order sums (row-major z, y, x).  Formulas:
+   kinetic      0.5 <|u|^2>              program/lbm_iht_xy.f90:777-821
+   dissipation  0.5 nu1 <(d_a u_b + d_b u_a)^2>  program/turbulence_hit_xy.f90:304-320,426-463
+   interface    sum |grad phi|           program/drop_analysis.f90:445-474
+   free energy  sum phi T ln(phi/(1-b phi)) - a phi^2   program/drop_analysis.f90:400-411 */
+int ora_reduce(ora_lattice* L, drp_stats* st) {
+    const drp_config* c = &L->c;
+    /* fresh stencils of the current fields */
+    for (int z = 0; z < c->nz; z++)
+        for (int y = 0; y < c->ny; y++)
+            for (int x = 0; x < c->nx; x++) stencils_at(L, x, y, z);
+    memset(st, 0, sizeof(*st));
+    double mass = 0, ke = 0, eps = 0, area = 0, fe = 0, pm = 0, m0 = 0, m1 = 0, m2 = 0;
+    for (int64_t s = 0; s < L->n; s++) {
+        double phi = L->phi[s];
+        mass += phi;
+        ke += 0.5 * (L->u1[s] * L->u1[s] + L->u2[s] * L->u2[s] + L->u3[s] * L->u3[s]);
+        double e = 0.0;
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) {
+                double sab = L->gu[(a * 3 + b) * L->n + s] + L->gu[(b * 3 + a) * L->n + s];
+                e += sab * sab;
+            }
+        eps += e;
+        double g0 = L->gp[s], g1 = L->gp[L->n + s], g2 = L->gp[2 * L->n + s];
+        area += sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        fe += phi * c->T * log(phi / (1.0 - c->b * phi)) - c->a * phi * phi;
+        pm += L->p[s];
+        m0 += L->u1[s];
+        m1 += L->u2[s];
+        m2 += L->u3[s];
+    }
+    double nn = (double)L->n;
+    st->phi_mass = mass;
+    st->phi_min = L->phi_min;
+    st->phi_max = L->phi_max;
+    st->kinetic = ke / nn;
+    st->dissipation = eps / nn * 0.5 * c->nu1;
+    st->interface_area = area;
+    st->free_energy = fe;
+    st->p_mean = pm / nn;
+    st->momentum[0] = m0;
+    st->momentum[1] = m1;
+    st->momentum[2] = m2;
+    return 0;
+}
+
+/* ---------------------------------------------------------------- deformation */
+
+/* Plane accessor for the deformation routines.  The reference measures in the
+   (x, z) plane at y = ymax/2 (walls normal to z).  `ax` selects the wall-normal
+   axis (DRP_WALL_Z: plane (x, z) with the third axis y; DRP_WALL_Y: plane (x, y)
+   with the third axis z). Indices wrap periodically. */
+typedef struct {
+    const ora_lattice* L;
+    int ax, third;
+    int n1, n2;
+} plane_t;
+
+static double plane_phi(const plane_t* P, int i1, int i2) {
+    const ora_lattice* L = P->L;
+    i1 = ((i1 % P->n1) + P->n1) % P->n1;
+    i2 = ((i2 % P->n2) + P->n2) % P->n2;
+    if (P->ax == DRP_WALL_Z) return L->phi[idx(L, i1, P->third, i2)];
+    return L->phi[idx(L, i1, i2, P->third)];
+}
+
+static double interp(const plane_t* P, double xj, double zj) {
+    int x1 = (int)xj, z1 = (int)zj; /* int(): truncation toward zero, dropch.f90:908-911 */
+    int x2 = x1 + 1, z2 = z1 + 1;
+    return ((x2 - xj) * (z2 - zj) * plane_phi(P, x1, z1) + (xj - x1) * (z2 - zj) * plane_phi(P, x2, z1)
+            + (x2 - xj) * (zj - z1) * plane_phi(P, x1, z2) + (xj - x1) * (zj - z1) * plane_phi(P, x2, z2))
+           / (((double)x2 - (double)x1) * ((double)z2 - (double)z1));
+}
+
+/* break_up3, prototype_program/two-phase/dropch.f90:857-948 */
+static int deformation_breakup3(const ora_lattice* L, double* Lout, double* Bout, double* Dout) {
+    const drp_config* c = &L->c;
+    int ax = c->wall_axis == DRP_WALL_Y ? DRP_WALL_Y : DRP_WALL_Z;
+    int n1 = c->nx, n2 = (ax == DRP_WALL_Z) ? c->nz : c->ny, n3 = (ax == DRP_WALL_Z) ? c->ny : c->nz;
+    const double pi = acos(-1.0);
+    const double mid = 0.5 * (c->phi1 + c->phi2);
+    plane_t P = {L, ax, (n3 - 1) / 2, n1, n2}; /* centroid plane ymax/2, :860 */
+    double xg = 0, zg = 0, w = 0;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+            double v = plane_phi(&P, i, j) - c->phi1;
+            xg = xg + (double)i * v;
+            zg = zg + (double)j * v;
+            w = w + v;
+        }
+    xg = xg / w;
+    zg = zg / w;
+    P.third = n3 / 2; /* search plane (ymax+1)/2, :882 */
+    double dist = 0.0;
+    int xt = 0, zt = 0;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++)
+            if (plane_phi(&P, i, j) >= mid) {
+                double r = sqrt(((double)i - xg) * ((double)i - xg) + ((double)j - zg) * ((double)j - zg));
+                if (dist <= r) {
+                    dist = r;
+                    xt = i;
+                    zt = j;
+                }
+            }
+    if (dist == 0.0) return DRP_ERR_STATE;
+    double th = acos(fabs(((double)xt - xg) / dist));
+    double dx = (double)xt - xg, dz = (double)zt - zg;
+    if (dx > 0.0 && dz > 0.0) th = th;
+    else if (dx < 0.0 && dz > 0.0) th = pi - th;
+    else if (dx < 0.0 && dz < 0.0) th = pi + th;
+    else if (dx > 0.0 && dz < 0.0) th = 2.0 * pi - th;
+    double B1 = 0.0, L1 = 0.0;
+    double den = tan(th) * sin(th) + cos(th);
+    for (int k = 1; k < 100000; k++) { /* short axis, :898-922 */
+        double zj = (0.1 * (double)k) / den;
+        double xj = -zj * tan(th);
+        xj = xj + xg;
+        zj = zj + zg;
+        double ph = interp(&P, xj, zj);
+        if (ph >= mid) B1 = sqrt((xj - xg) * (xj - xg) + (zj - zg) * (zj - zg)) * 2.0;
+        if (ph < mid) break;
+    }
+    for (int k = 1; k < 100000; k++) { /* long axis, :924-946 */
+        double xj = (0.1 * (double)k) / den;
+        double zj = xj * tan(th);
+        xj = xj + xg;
+        zj = zj + zg;
+        double ph = interp(&P, xj, zj);
+        if (ph >= mid) L1 = sqrt((xj - xg) * (xj - xg) + (zj - zg) * (zj - zg)) * 2.0;
+        if (ph < mid) break;
+    }
+    *Lout = L1;
+    *Bout = B1;
+    *Dout = (L1 - B1) / (L1 + B1); /* :948 */
+    return 0;
+}
+
+/* area variant: L = 2 max r, B = 4 area/(pi L), prototype_program/two-phase/drop.f90:439-463 */
+static int deformation_area(const ora_lattice* L, double* Lout, double* Bout, double* Dout) {
+    const drp_config* c = &L->c;
+    int ax = c->wall_axis == DRP_WALL_Y ? DRP_WALL_Y : DRP_WALL_Z;
+    int n1 = c->nx, n2 = (ax == DRP_WALL_Z) ? c->nz : c->ny, n3 = (ax == DRP_WALL_Z) ? c->ny : c->nz;
+    const double pi = acos(-1.0);
+    const double mid = 0.5 * (c->phi1 + c->phi2);
+    plane_t P = {L, ax, n3 / 2, n1, n2};
+    double xg = 0, zg = 0, cnt = 0;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++)
+            if (plane_phi(&P, i, j) >= mid) {
+                xg += i;
+                zg += j;
+                cnt += 1.0;
+            }
+    if (cnt == 0.0) return DRP_ERR_STATE;
+    xg /= cnt;
+    zg /= cnt;
+    double rmax = 0.0;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++)
+            if (plane_phi(&P, i, j) >= mid) {
+                double r = sqrt((i - xg) * (i - xg) + (j - zg) * (j - zg));
+                if (r > rmax) rmax = r;
+            }
+    double Lx = 2.0 * rmax;
+    double Bx = 4.0 * cnt / (pi * Lx);
+    *Lout = Lx;
+    *Bout = Bx;
+    *Dout = (Lx - Bx) / (Lx + Bx);
+    return 0;
+}
+
+int ora_deformation(const ora_lattice* L, int mode, double* Lout, double* Bout, double* Dout) {
+    if (mode == 0) return deformation_breakup3(L, Lout, Bout, Dout);
+    if (mode == 1) return deformation_area(L, Lout, Bout, Dout);
+    return DRP_ERR_ARG;
+}

@@ -1,0 +1,137 @@
+"""ctypes wrapper of the CPU restatement (oracle/lbm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from droplet_amd.abi import Config, Drop, Stats
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liblbm_oracle.so")
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        L.ora_create.restype = vp
+        L.ora_create.argtypes = [C.POINTER(Config)]
+        L.ora_destroy.argtypes = [vp]
+        L.ora_set_threads.argtypes = [vp, C.c_int]
+        L.ora_step_count.restype = C.c_int64
+        L.ora_step_count.argtypes = [vp]
+        L.ora_init_droplets.argtypes = [vp, C.POINTER(Drop), C.c_int, C.c_double]
+        L.ora_init_fields.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.ora_step.argtypes = [vp, C.c_int]
+        L.ora_get_macro.argtypes = [vp, C.c_int, dp]
+        L.ora_get_populations.argtypes = [vp, C.c_int, dp]
+        L.ora_set_populations.argtypes = [vp, C.c_int, dp]
+        L.ora_inject_momentum.argtypes = [vp, dp]
+        L.ora_set_force.argtypes = [vp, dp, dp, dp]
+        L.ora_phi_range.argtypes = [vp, dp, dp]
+        L.ora_reduce.argtypes = [vp, C.POINTER(Stats)]
+        L.ora_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleLattice:
+    """Same call surface as droplet_amd.Lattice, computed by the CPU restatement."""
+
+    def __init__(self, cfg: Config, threads: int | None = None):
+        self.cfg = cfg
+        self.shape = (cfg.nz, cfg.ny, cfg.nx)
+        self.n = cfg.nx * cfg.ny * cfg.nz
+        self._h = lib().ora_create(C.byref(cfg))
+        if not self._h:
+            raise ValueError("ora_create failed")
+        if threads is None:
+            threads = int(os.environ.get("DROPLET_ORACLE_THREADS", min(8, os.cpu_count() or 1)))
+        lib().ora_set_threads(self._h, threads)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().ora_destroy(h)
+            self._h = None
+
+    def init_droplets(self, drops, p_init: float = 1.0 / 3.0):
+        arr = (Drop * len(drops))(*[Drop(*d) for d in drops])
+        lib().ora_init_droplets(self._h, arr, len(drops), p_init)
+
+    def init_fields(self, phi, p, u1, u2, u3):
+        fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (phi, p, u1, u2, u3)]
+        lib().ora_init_fields(self._h, *[_dp(a) for a in fs])
+
+    def step(self, n: int = 1):
+        lib().ora_step(self._h, int(n))
+
+    @property
+    def step_count(self) -> int:
+        return int(lib().ora_step_count(self._h))
+
+    def macro(self, field: int) -> np.ndarray:
+        out = np.empty(self.shape, np.float64)
+        if lib().ora_get_macro(self._h, field, _dp(out)) != 0:
+            raise ValueError(field)
+        return out
+
+    def fields(self) -> dict:
+        return {k: self.macro(i) for i, k in enumerate(("phi", "p", "u1", "u2", "u3"))}
+
+    def populations(self, which: int) -> np.ndarray:
+        out = np.empty((15,) + self.shape, np.float64)
+        lib().ora_get_populations(self._h, which, _dp(out))
+        return out
+
+    def set_populations(self, which: int, pops: np.ndarray):
+        a = np.ascontiguousarray(pops, np.float64)
+        assert a.shape == (15,) + self.shape
+        lib().ora_set_populations(self._h, which, _dp(a))
+
+    def inject_momentum(self, g: np.ndarray):
+        a = np.ascontiguousarray(g, np.float64)
+        assert a.shape == (15,) + self.shape
+        lib().ora_inject_momentum(self._h, _dp(a))
+
+    def set_force(self, fx, fy, fz):
+        fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (fx, fy, fz)]
+        lib().ora_set_force(self._h, *[_dp(a) for a in fs])
+
+    def phi_range(self):
+        mn, mx = C.c_double(), C.c_double()
+        lib().ora_phi_range(self._h, C.byref(mn), C.byref(mx))
+        return mn.value, mx.value
+
+    def reduce(self) -> dict:
+        st = Stats()
+        lib().ora_reduce(self._h, C.byref(st))
+        return st.as_dict()
+
+    def deformation(self, mode: int = 0):
+        L, B, D = C.c_double(), C.c_double(), C.c_double()
+        rc = lib().ora_deformation(self._h, mode, C.byref(L), C.byref(B), C.byref(D))
+        if rc != 0:
+            raise ValueError(f"deformation failed ({rc})")
+        return L.value, B.value, D.value

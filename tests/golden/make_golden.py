@@ -1,0 +1,222 @@
+"""Generate the golden-vector fixtures of this directory from the compiled reference.
+
+Run in the development container only (needs /root/reference, amdflang and
+MPICH):  python tests/golden/make_golden.py
+
+Each case builds a parameter-patched copy of a reference program with
+oracle/ref_build.py, runs it under mpiexec with OMP_NUM_THREADS=1 (the reference
+has OpenMP races on module variables, so threads would make it nondeterministic),
+reads the Fortran unformatted output files it writes and stores inputs, outputs
+and the parameter set as a compressed .npz.  Nothing of the reference itself is
+stored: only numbers it produced.
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from oracle import ref_build  # noqa: E402
+
+MPIEXEC = "/opt/conda/bin/mpiexec"
+CX = np.array([0, 1, 0, 0, -1, 0, 0, 1, -1, 1, 1, -1, 1, -1, -1])
+CY = np.array([0, 0, 1, 0, 0, -1, 0, 1, 1, -1, 1, -1, -1, 1, -1])
+CZ = np.array([0, 0, 0, 1, 0, 0, -1, 1, 1, 1, -1, -1, -1, -1, 1])
+E = np.array([2 / 9] + [1 / 9] * 6 + [1 / 72] * 8)
+
+
+# ------------------------------------------------------------------ Fortran I/O
+def read_records(path: str, nval: int) -> np.ndarray:
+    """Fortran unformatted sequential file of equal records of nval doubles."""
+    raw = np.fromfile(path, dtype=np.uint8)
+    rec = 8 * nval + 8
+    if raw.size % rec:
+        raise ValueError(f"{path}: size {raw.size} is not a multiple of {rec}")
+    r = raw.reshape(-1, rec)
+    head = r[:, :4].copy().view(np.int32).ravel()
+    tail = r[:, -4:].copy().view(np.int32).ravel()
+    if not ((head == 8 * nval).all() and (tail == 8 * nval).all()):
+        raise ValueError(f"{path}: bad record markers")
+    return r[:, 4:-4].copy().view(np.float64).reshape(-1, nval)
+
+
+def write_records(path: str, values: np.ndarray) -> None:
+    """One double per record (the g-only input format of program/lbm_drop_xy.f90:799)."""
+    v = np.ascontiguousarray(values, np.float64).ravel()
+    out = np.empty((v.size, 16), np.uint8)
+    marker = np.array([8], np.int32).view(np.uint8)
+    out[:, :4] = marker
+    out[:, 4:12] = v.view(np.uint8).reshape(-1, 8)
+    out[:, 12:] = marker
+    out.tofile(path)
+
+
+def fnum(v: float) -> str:
+    """Fortran double literal of a Python float (shortest round-trip digits)."""
+    s = repr(float(v))
+    return s.replace("e", "d") if "e" in s else s + "d0"
+
+
+def run(exe: str, nranks: int, cwd: str) -> None:
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([MPIEXEC, "-n", str(nranks), exe], cwd=cwd, env=env, check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+
+
+# ------------------------------------------------------------------ periodic program
+def periodic_params(N, step, eta, Re, Ca, Dv, Dr, umax, step_input=10 ** 8):
+    return dict(
+        xmax=str(N - 1), ymax=str(N - 1), zmax=str(N - 1), step=str(step),
+        step_input=str(step_input), step_input_file_num="0",
+        datadir_input='"./in/"', datadir_output='"./out/"', datadir_output_fg='"./out/fg/"',
+        step_output="1", step_putput_fg="10",
+        Re=fnum(Re), eta=fnum(eta), Ca=fnum(Ca), D_vortex=fnum(Dv),
+        D_ratio=fnum(Dr), umax=fnum(umax),
+        xc="0.5d0*dble(xmax)", yc="0.5d0*dble(ymax)", zc="0.5d0*dble(zmax)")
+
+
+def periodic_phys(N, eta, Re, Ca, Dv, Dr, umax):
+    """Constants exactly as program/lbm_drop_xy.f90:40-51 derives them."""
+    D = Dr * Dv
+    nu1 = umax * Dv * Dr / Re
+    nu2 = eta * nu1
+    sigma = nu1 * umax / Ca
+    pi = math.acos(-1.0)
+    return dict(nx=N, ny=N, nz=N, nu1=nu1, nu2=nu2, kappa_g=(sigma / 1.7039) ** (1.0 / 0.9991),
+                tau_f=0.7, kappa_f=0.01, drop=[0.5 * (N - 1)] * 3 + [D], p_init=1.0 / 3.0,
+                force_amp=2.0 * pi * pi * umax * umax / (Dv * Re), force_k=pi / Dv)
+
+
+def assemble_periodic(outdir: str, step: int, N: int, nranks: int = 8) -> np.ndarray:
+    """output() files 0_<rank>_<n>.bin, records (phi,u1,u2,u3,p), z-y-x loops, x split over ranks."""
+    xp = N // nranks
+    out = np.zeros((5, N, N, N))
+    for r in range(nranks):
+        a = read_records(os.path.join(outdir, f"0_{r}_{step}.bin"), 5).reshape(N, N, xp, 5)
+        out[:, :, :, r * xp:(r + 1) * xp] = np.moveaxis(a, -1, 0)
+    # reorder (phi,u1,u2,u3,p) -> (phi,p,u1,u2,u3)
+    return out[[0, 4, 1, 2, 3]]
+
+
+def assemble_pops(outdir: str, step: int, N: int, nranks: int = 8) -> np.ndarray:
+    """outputfg() files, records (f_i, g_i), i fastest then x, y, z."""
+    xp = N // nranks
+    f = np.zeros((15, N, N, N))
+    g = np.zeros((15, N, N, N))
+    for r in range(nranks):
+        a = read_records(os.path.join(outdir, "fg", f"0_{r}_{step}_fg.bin"), 2).reshape(N, N, xp, 15, 2)
+        f[:, :, :, r * xp:(r + 1) * xp] = np.moveaxis(a[..., 0], -1, 0)
+        g[:, :, :, r * xp:(r + 1) * xp] = np.moveaxis(a[..., 1], -1, 0)
+    return np.stack([f, g])
+
+
+def case_periodic(name, N, steps, keep, eta, Re, Ca, Dv, Dr, umax, pops_at=None):
+    params = periodic_params(N, steps, eta, Re, Ca, Dv, Dr, umax)
+    exe = ref_build.build("program/lbm_drop_xy.f90", f"lbm_drop_xy_{name}", params)
+    with tempfile.TemporaryDirectory() as tmp:
+        run(exe, 8, tmp)
+        data = {f"fields_{n}": assemble_periodic(os.path.join(tmp, "out"), n, N) for n in keep}
+        if pops_at is not None:
+            data[f"pops_{pops_at}"] = assemble_pops(os.path.join(tmp, "out"), pops_at, N)
+    meta = dict(case=name, program="program/lbm_drop_xy.f90", nranks=8, params=params,
+                phys=periodic_phys(N, eta, Re, Ca, Dv, Dr, umax), steps=list(keep),
+                sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "program/lbm_drop_xy.f90")))
+    return data, meta
+
+
+def case_injection(name, N, steps, keep, step_input, seed, eta, Re, Ca, Dv, Dr, umax):
+    """HIT-style injection at step_input (program/lbm_drop_xy.f90:789-828) plus the
+    Taylor-Green body force computed at that step (externalforce :668-693)."""
+    params = periodic_params(N, steps, eta, Re, Ca, Dv, Dr, umax, step_input=step_input)
+    params["step_input_file_num"] = "7"
+    exe = ref_build.build("program/lbm_drop_xy.f90", f"lbm_drop_xy_{name}", params)
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    k = 2 * np.pi / N
+    u1 = 0.02 * np.sin(k * x) * np.cos(k * z) + 0.002 * rng.standard_normal((N, N, N))
+    u2 = 0.002 * rng.standard_normal((N, N, N))
+    u3 = -0.02 * np.cos(k * x) * np.sin(k * z) + 0.002 * rng.standard_normal((N, N, N))
+    p = 1.0 / 3.0 + 1e-3 * rng.standard_normal((N, N, N))
+    g = np.empty((15, N, N, N))
+    for i in range(15):
+        cu = CX[i] * u1 + CY[i] * u2 + CZ[i] * u3
+        g[i] = E[i] * (3 * p + 3 * cu + 4.5 * cu * cu - 1.5 * (u1 * u1 + u2 * u2 + u3 * u3))
+    xp = N // 8
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "in"))
+        for r in range(8):
+            sub = g[:, :, :, r * xp:(r + 1) * xp]  # [i][z][y][x] -> records i fastest, then x, y, z
+            write_records(os.path.join(tmp, "in", f"0_{r}_7_fg.bin"), np.moveaxis(sub, 0, -1))
+        run(exe, 8, tmp)
+        data = {f"fields_{n}": assemble_periodic(os.path.join(tmp, "out"), n, N) for n in keep}
+    data["g_injected"] = g
+    meta = dict(case=name, program="program/lbm_drop_xy.f90", nranks=8, params=params,
+                phys=periodic_phys(N, eta, Re, Ca, Dv, Dr, umax), steps=list(keep), step_input=step_input,
+                sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "program/lbm_drop_xy.f90")))
+    return data, meta
+
+
+def case_walls(name, N, steps, D, nu, uw, kappag):
+    """Moving-wall shear droplet, prototype_program/two-phase/dropch.f90 (walls normal to z).
+    Nx = 3 is hard-coded (:146) and the field gather to rank 0 names ranks 0..8
+    explicitly (:1103-1880), so the program runs on 9 ranks (Ny = 3); with Ny >= 2
+    both walls are applied (with Ny = 1 the `else if (group_y == Ny-1)` branch of
+    :684 never runs)."""
+    assert steps % 30000 == 0, "dropch writes fields only when mod(n,30000) == 0 (:1905)"
+    params = dict(xmax=str(N - 1), ymax=str(N - 1), zmax=str(N - 1), step=str(steps), start="1",
+                  datadir='"./out/"', D=fnum(D) + " * ds", nu1=fnum(nu), nu2=fnum(nu),
+                  uw=fnum(uw), kappag=fnum(kappag))
+    exe = ref_build.build("prototype_program/two-phase/dropch.f90", f"dropch_{name}", params)
+    with tempfile.TemporaryDirectory() as tmp:
+        run(exe, 9, tmp)
+        rec = read_records(os.path.join(tmp, "out", f"{steps}.bin"), 8).reshape(N, N, N, 8)
+    xyz = rec[..., :3]
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    assert (xyz[..., 0] == x).all() and (xyz[..., 1] == y).all() and (xyz[..., 2] == z).all()
+    data = {f"phi_{steps}": rec[..., 3], f"p_{steps}": rec[..., 7]}
+    meta = dict(case=name, program="prototype_program/two-phase/dropch.f90", nranks=9, params=params,
+                phys=dict(nx=N, ny=N, nz=N, nu1=nu, nu2=nu, tau_f=0.7, kappa_f=0.01, kappa_g=kappag,
+                          wall_u=uw, wall_start=1, drop=[0.5 * (N - 1)] * 3 + [D], p_init=0.0),
+                steps=[steps],
+                sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "prototype_program/two-phase/dropch.f90")))
+    return data, meta
+
+
+def save(name, data, meta):
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, meta=np.array(json.dumps(meta)), **data)
+    print(f"wrote {path} ({os.path.getsize(path) / 1e6:.2f} MB)")
+
+
+CASES = {
+    # 16^3, viscosity ratio 2 (nu(phi) depends on the global phi range every step)
+    "periodic16": lambda: case_periodic("periodic16", 16, 200, (1, 10, 200), eta=2.0, Re=0.8, Ca=0.1,
+                                        Dv=16.0, Dr=0.5, umax=0.01, pops_at=10),
+    # 32^3, equal viscosities, 2000 steps (the step count of the 64^3 shear case)
+    "periodic32": lambda: case_periodic("periodic32", 32, 2000, (2000,), eta=1.0, Re=1.2, Ca=0.05,
+                                        Dv=32.0, Dr=0.375, umax=0.01),
+    # injected momentum populations + Taylor-Green body force from step 3 on
+    "inject16": lambda: case_injection("inject16", 16, 60, (3, 4, 60), step_input=3, seed=16, eta=1.0,
+                                       Re=0.4, Ca=0.1, Dv=16.0, Dr=0.5, umax=0.02),
+    # moving walls normal to z, 24^3, 30000 steps
+    "walls24": lambda: case_walls("walls24", 24, 30000, D=10.0, nu=0.1, uw=0.01, kappag=0.01),
+}
+
+
+def main():
+    if not ref_build.available():
+        sys.exit("reference sources / amdflang / MPICH not available")
+    for name in (sys.argv[1:] or list(CASES)):
+        save(name, *CASES[name]())
+
+
+if __name__ == "__main__":
+    main()

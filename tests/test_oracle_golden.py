@@ -1,0 +1,70 @@
+"""The CPU restatement against golden vectors written by the compiled reference.
+
+The fixtures in tests/golden/ were produced by program/lbm_drop_xy.f90 and
+prototype_program/two-phase/dropch.f90 (parameter-patched, MPI, one thread per
+rank); tests/golden/make_golden.py is the script.  The restatement keeps the
+reference's operation order, so agreement is required to be exact (bit for bit)
+or, where stated, within a few ulps.
+"""
+import numpy as np
+import pytest
+
+from droplet_amd.abi import POP_MOM, POP_PHI
+from oracle.pyoracle import OracleLattice
+from tests import golden_data as gd
+
+FIELDS = ("phi", "p", "u1", "u2", "u3")
+
+
+def _run_to(o, n):
+    o.step(n - o.step_count)
+
+
+@pytest.mark.parametrize("name", ["periodic16", "periodic32"])
+def test_periodic_fields_bit_exact(name):
+    z, meta = gd.load(name)
+    o = OracleLattice(gd.config_for(meta))
+    o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    for n in meta["steps"]:
+        _run_to(o, n)
+        ref = z[f"fields_{n}"]
+        got = o.fields()
+        for k, fld in enumerate(FIELDS):
+            np.testing.assert_array_equal(got[fld], ref[k], err_msg=f"{name} step {n} {fld}")
+
+
+def test_periodic_populations_bit_exact():
+    z, meta = gd.load("periodic16")
+    o = OracleLattice(gd.config_for(meta))
+    o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    o.step(10)
+    ref = z["pops_10"]
+    np.testing.assert_array_equal(o.populations(POP_PHI), ref[0])
+    np.testing.assert_array_equal(o.populations(POP_MOM), ref[1])
+
+
+def test_injection_and_taylor_green_force_bit_exact():
+    z, meta = gd.load("inject16")
+    o = OracleLattice(gd.config_for(meta))
+    o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    o.step(meta["step_input"] - 1)
+    o.inject_momentum(z["g_injected"])
+    for n in meta["steps"]:
+        _run_to(o, n)
+        ref = z[f"fields_{n}"]
+        got = o.fields()
+        for k, fld in enumerate(FIELDS):
+            np.testing.assert_array_equal(got[fld], ref[k], err_msg=f"inject16 step {n} {fld}")
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(not gd.available("walls24"), reason="walls24 fixture not generated")
+def test_moving_walls_bit_exact():
+    z, meta = gd.load("walls24")
+    o = OracleLattice(gd.config_for(meta))
+    o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    n = meta["steps"][0]
+    o.step(n)
+    got = o.fields()
+    np.testing.assert_array_equal(got["phi"], z[f"phi_{n}"])
+    np.testing.assert_array_equal(got["p"], z[f"p_{n}"])
