@@ -1,0 +1,176 @@
+"""Benchmark of the D3Q15 two-phase time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (one "step" = one lattice Boltzmann time step over the whole grid):
+256^3 shear droplet — walls normal to y at y=0 and y=255 moving -U/+U along x,
+periodic in x and z, one droplet of radius 40 at the centre (reference sharp
+step), tau_f = 0.7, nu = 0.1 (tau_g = 0.8), kappa_f = 0.01, kappa_g from
+sigma = 1.7039 kappa_g^0.9991, wall speed from Ca = 0.2 (Ca = U nu D / (H sigma)).
+Populations and fields are resident in HBM before the timed region starts.
+
+Prints one JSON line: MLUPS of the FP64 path (value), the HBM roofline of the
+step kernel (algorithmic 560 B per lattice update over its HIP-event time),
+and the CPU baseline measured on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "D3Q15 two-phase MLUPS & %HBM peak at 256^3; FP32-shifted vs FP64 @ stated err"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LU = {"f64": 560, "f32": 280}  # 2 sets x 15 x (read + write) + 5 fields x (write + read)
+
+
+def shear_config(N=256, precision=0, Ca=0.2):
+    from droplet_amd.abi import DRP_WALL_Y, kappa_g_from_sigma, make_config
+    kappa_g = 0.002
+    sigma = 1.7039 * kappa_g ** 0.9991
+    nu, D, H = 0.1, 80.0, float(N - 1)
+    U = Ca * H * sigma / (nu * D)
+    cfg = make_config(N, nu1=nu, nu2=nu, tau_f=0.7, kappa_f=0.01, kappa_g=kappa_g,
+                      wall_axis=DRP_WALL_Y, wall_u=U, wall_start=1, precision=precision)
+    return cfg, [(0.5 * (N - 1),) * 3 + (D,)], dict(Ca=Ca, U=U, sigma=sigma, kappa_g=kappa_g, radius=D / 2)
+
+
+def traffic_from_profiles(kernel_tag: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), if recorded."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            return float(json.load(fh)["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def run_gpu(cfg, drops, steps, warmup):
+    from droplet_amd.lattice import Lattice
+    lat = Lattice(cfg)
+    lat.init_droplets(drops, 1.0 / 3.0)
+    if warmup:
+        lat.step(warmup)
+    lat.sync()
+    t0 = time.perf_counter()
+    ms_events = lat.time_steps(steps)   # HIP events on the lattice's own stream
+    lat.sync()
+    wall = time.perf_counter() - t0
+    return lat, wall, ms_events
+
+
+def cpu_reference_mlups():
+    """Compiled reference (oracle/_ref), 64^3 periodic droplet, 8 MPI ranks x 1 thread,
+    timed from its own cumulative time.d log between steps 100 and 300."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "lbm_drop_xy_bench64")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([mpiexec, "-n", "8", exe], cwd=tmp, env=env, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.PIPE, timeout=300)
+        if r.returncode != 0 or not os.path.exists(os.path.join(tmp, "time.d")):
+            return None
+        rows = [ln.split() for ln in open(os.path.join(tmp, "time.d")) if ln.strip()]
+    t = {int(float(a[0])): float(a[1]) for a in rows}
+    if 100 not in t or 300 not in t:
+        return None
+    mlups = 64 ** 3 * 200 / (t[300] - t[100]) / 1e6
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": 8, "kind": "reference",
+            "sample": "program/lbm_drop_xy.f90 compiled from source (amdflang -O2, MPICH), 64^3 periodic "
+                      "droplet, 8 MPI ranks x 1 OpenMP thread, steps 100-300 from its own time.d"}
+
+
+def cpu_port_mlups():
+    """CPU restatement (oracle/), 64^3 shear droplet, a few steps on 8 threads."""
+    from oracle.pyoracle import OracleLattice
+    cfg, drops, _ = shear_config(64)
+    threads = 8
+    o = OracleLattice(cfg, threads=threads)
+    o.init_droplets([(31.5, 31.5, 31.5, 20.0)])
+    o.step(1)
+    n = 4
+    t0 = time.perf_counter()
+    o.step(n)
+    dt = time.perf_counter() - t0
+    return {"value": round(64 ** 3 * n / dt / 1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"C restatement (oracle/lbm_oracle.c), 64^3 shear droplet, {n} steps, {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: the whole domain runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    N = args.grid
+    cfg, drops, phys = shear_config(N)
+    nodes = N ** 3
+    lat, wall, ms = run_gpu(cfg, drops, args.steps, args.warmup)
+    mlups = nodes * args.steps / wall / 1e6
+    kernel_ms = ms / args.steps
+    achieved = BYTES_PER_LU["f64"] * nodes / (kernel_ms * 1e-3) / 1e9
+    stats = lat.reduce()
+    tile = lat.tile_shape
+    del lat
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_reference_mlups()
+        except Exception:
+            cpu = None
+        if cpu is None:
+            try:
+                cpu = cpu_port_mlups()
+            except Exception:
+                cpu = None
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"shear droplet {N}^3, walls normal to y, radius {phys['radius']:.0f}, "
+                               f"Ca {phys['Ca']}, FP64 populations",
+                   "grid": [N, N, N], "wall_u": round(phys["U"], 6), "kappa_g": phys["kappa_g"],
+                   "tau_f": 0.7, "nu": 0.1, "tile_shape": tile},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"step_fp64_{N}"),
+                     "kernel_ms": round(kernel_ms, 5), "bytes_per_lu": BYTES_PER_LU["f64"]},
+        "cpu_baseline": cpu,
+        "phi_mass": stats["phi_mass"],
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

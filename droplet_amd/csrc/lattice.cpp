@@ -446,3 +446,58 @@ extern "C" int drp_deformation(drp_lattice* L, int mode, double* Lo, double* Bo,
     if (mode != 0 && mode != 1) return drp_fail(DRP_ERR_ARG, "unknown deformation mode %d", mode);
     return deformation(L, mode, Lo, Bo, Do);
 }
+
+// -------------------------------------------------------------------- measurement / tuning
+
+extern "C" int drp_time_steps(drp_lattice* L, int nsteps, double* ms) {
+    if (!L || !ms) return drp_fail(DRP_ERR_ARG, "null argument");
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, L->stream));
+    int rc = drp_step(L, nsteps);
+    HIP_TRY(hipEventRecord(e1, L->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t;
+    return rc;
+}
+
+extern "C" int drp_tile_shape_count(void) { return SHAPE_COUNT; }
+
+extern "C" int drp_set_tile_shape(drp_lattice* L, int shape) {
+    if (!L) return drp_fail(DRP_ERR_ARG, "null lattice");
+    if (shape < 0 || shape >= SHAPE_COUNT) return drp_fail(DRP_ERR_ARG, "tile shape %d out of range", shape);
+    L->shape = shape;
+    return DRP_OK;
+}
+
+extern "C" int drp_get_tile_shape(const drp_lattice* L) { return L ? L->shape : -1; }
+
+extern "C" int drp_autotune(const drp_config* cfg, int nsteps, int* best, double* ms_out) {
+    if (!cfg || !best || nsteps < 1) return drp_fail(DRP_ERR_ARG, "bad autotune arguments");
+    drp_lattice* L = nullptr;
+    int rc = drp_create(cfg, &L);
+    if (rc) return rc;
+    drp_drop d{0.5 * (cfg->nx - 1), 0.5 * (cfg->ny - 1), 0.5 * (cfg->nz - 1), 0.3 * cfg->nx};
+    rc = drp_init_droplets(L, &d, 1, 1.0 / 3.0);
+    double bestms = 1e300;
+    for (int s = 0; s < SHAPE_COUNT && !rc; s++) {
+        L->shape = s;
+        double ms = 0.0;
+        rc = drp_time_steps(L, 2, &ms);
+        if (!rc) rc = drp_time_steps(L, nsteps, &ms);
+        if (rc) break;
+        ms /= nsteps;
+        if (ms_out) ms_out[s] = ms;
+        if (ms < bestms) {
+            bestms = ms;
+            *best = s;
+        }
+    }
+    drp_destroy(L);
+    return rc;
+}

@@ -56,6 +56,11 @@ def lib():
         L.drp_set_force.argtypes = [vp, dp, dp, dp, sz]
         L.drp_reduce.argtypes = [vp, C.POINTER(Stats)]
         L.drp_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.drp_time_steps.argtypes = [vp, C.c_int, dp]
+        L.drp_tile_shape_count.argtypes = []
+        L.drp_set_tile_shape.argtypes = [vp, C.c_int]
+        L.drp_get_tile_shape.argtypes = [vp]
+        L.drp_autotune.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_int), dp]
         _lib = L
     return _lib
 
@@ -115,6 +120,20 @@ class Lattice:
     def sync(self):
         _check(lib().drp_sync(self._h), "drp_sync")
 
+    def time_steps(self, n: int) -> float:
+        """Run n steps between HIP events on the handle's stream; returns milliseconds."""
+        ms = C.c_double()
+        _check(lib().drp_time_steps(self._h, int(n), C.byref(ms)), "drp_time_steps")
+        return ms.value
+
+    @property
+    def tile_shape(self) -> int:
+        return int(lib().drp_get_tile_shape(self._h))
+
+    @tile_shape.setter
+    def tile_shape(self, shape: int):
+        _check(lib().drp_set_tile_shape(self._h, int(shape)), "drp_set_tile_shape")
+
     @property
     def step_count(self) -> int:
         return int(lib().drp_step_count(self._h))
@@ -159,3 +178,16 @@ class Lattice:
         L, B, D = C.c_double(), C.c_double(), C.c_double()
         _check(lib().drp_deformation(self._h, int(mode), C.byref(L), C.byref(B), C.byref(D)), "drp_deformation")
         return L.value, B.value, D.value
+
+
+def tile_shape_count() -> int:
+    return int(lib().drp_tile_shape_count())
+
+
+def autotune(cfg: Config, nsteps: int = 10):
+    """Fastest step-kernel tile shape for cfg; returns (best, [ms per step per shape])."""
+    n = tile_shape_count()
+    best = C.c_int(-1)
+    ms = (C.c_double * n)()
+    _check(lib().drp_autotune(C.byref(cfg), int(nsteps), C.byref(best), ms), "drp_autotune")
+    return best.value, list(ms)

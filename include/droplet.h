@@ -157,6 +157,23 @@ int drp_reduce(drp_lattice* lat, drp_stats* stats);
  * mode 1: L = 2 max r, B = 4 area/(pi L) (prototype_program/two-phase/drop.f90:439-463). */
 int drp_deformation(drp_lattice* lat, int mode, double* L, double* B, double* D);
 
+/* ---- measurement and tuning (no reference counterpart: the reference times its
+ * loop with MPI_Wtime into time.d, program/lbm_drop_xy.f90:850-864) ---- */
+
+/* Run nsteps steps bracketed by HIP events recorded on the handle's own stream;
+ * *ms = elapsed device time of those steps.  Blocks until they finish. */
+int drp_time_steps(drp_lattice* lat, int nsteps, double* ms);
+
+/* Number of tile shapes of the step kernel, and selection of one for a handle. */
+int drp_tile_shape_count(void);
+int drp_set_tile_shape(drp_lattice* lat, int shape);
+int drp_get_tile_shape(const drp_lattice* lat);
+
+/* Time every tile shape on a temporary lattice of this configuration (a centred
+ * droplet, nsteps steps each after 2 warm-up steps) and return the fastest;
+ * ms_per_step (may be NULL) receives drp_tile_shape_count() timings. */
+int drp_autotune(const drp_config* cfg, int nsteps, int* best_shape, double* ms_per_step);
+
 /* Message for the calling thread's last failure ("" if none). */
 const char* drp_last_error(void);
 
