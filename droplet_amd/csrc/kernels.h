@@ -7,7 +7,9 @@
 namespace drp {
 
 enum { FORCE_NONE = 0, FORCE_TAYLOR_GREEN = 1, FORCE_FIELD = 2 };
-enum { SHAPE_16x4x4 = 0, SHAPE_32x4x2 = 1, SHAPE_64x2x2 = 2, SHAPE_32x2x4 = 3, SHAPE_COUNT = 4 };
+// tile shapes of the step kernel (nodes per workgroup in x, y, z; two threads per node)
+enum { SHAPE_16x4x4 = 0, SHAPE_32x4x2 = 1, SHAPE_64x2x2 = 2, SHAPE_32x2x4 = 3, SHAPE_32x4x4 = 4, SHAPE_32x8x2 = 5,
+       SHAPE_64x4x2 = 6, SHAPE_COUNT = 7 };
 
 // Physical constants of one lattice as the kernels need them.
 struct Phys {
@@ -36,11 +38,13 @@ struct StepArgs {
     const double *fx, *fy, *fz;           // FORCE_FIELD
     const double *sinx, *cosx, *sinz, *cosz;  // FORCE_TAYLOR_GREEN tables
     double tau_f, kf, kf6, kg, T, a, b, nu1, dnu, tg_amp, phi1, phi2;
+    double tau_const;     // tau_g when nu1 == nu2: 0.5 + 3 nu1
     double wall_u_now;
     int wall_axis;        // 0, 2 (y) or 3 (z)
     int force_kind;
     uint64_t* range;      // 3 slots x (max, min) as order-preserving bits
     int slot_read, slot_write, slot_reset;
+    int wrap_mod;         // set by the launcher: a grid extent does not exceed the tile extent
 };
 
 hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st);

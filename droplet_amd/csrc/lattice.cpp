@@ -41,6 +41,13 @@ int drp_fail(int code, const char* fmt, ...) {
 
 extern "C" const char* drp_last_error(void) { return g_last_error.c_str(); }
 
+extern "C" int drp_abi_sizes(size_t* config, size_t* drop, size_t* stats) {
+    if (config) *config = sizeof(drp_config);
+    if (drop) *drop = sizeof(drp_drop);
+    if (stats) *stats = sizeof(drp_stats);
+    return DROPLET_ABI_VERSION;
+}
+
 extern "C" int drp_config_default(drp_config* c) {
     if (!c) return drp_fail(DRP_ERR_ARG, "null config");
     std::memset(c, 0, sizeof(*c));
@@ -284,6 +291,7 @@ StepArgs make_step_args(drp_lattice* L, int64_t nstep) {
     a.b = P.b;
     a.nu1 = P.nu1;
     a.dnu = P.dnu;
+    a.tau_const = 0.5 + 3.0 * P.nu1;
     a.tg_amp = P.tg_amp;
     a.phi1 = P.phi1;
     a.phi2 = P.phi2;

@@ -31,9 +31,15 @@ template <int Q0, int Q1, typename Fn> __device__ __forceinline__ void static_fo
     }
 }
 
-__device__ __forceinline__ int wrapi(int v, int n) {
-    v %= n;
-    return v < 0 ? v + n : v;
+// periodic wrap of a staged halo coordinate.  When every grid extent exceeds
+// the tile extent the coordinate lies in [-n, 2n) and one conditional add or
+// subtract suffices; otherwise (tiny grids) the launcher sets wrap_mod.
+__device__ __forceinline__ int wrapi(int v, int n, bool mod) {
+    if (mod) {
+        v %= n;
+        return v < 0 ? v + n : v;
+    }
+    return v < 0 ? v + n : (v >= n ? v - n : v);
 }
 
 // Map the linear workgroup id to a tile so that each XCD (workgroups are dealt
@@ -44,17 +50,40 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-template <int TX, int TY, int TZ, bool WALL, int FORCE, bool TAUFIELD>
-__global__ __launch_bounds__(TX* TY* TZ) void step_fp64(StepArgs a) {
-    constexpr int NT = TX * TY * TZ;
+// Correctly rounded x / c for c in {3, 5, 10} with r = RN(1/c): q = RN(x r) is
+// faithful, so its remainder e = x - q c is exact in one FMA, and q + e r rounds
+// to RN(x / c) because x / c never lies closer than ulp/20 to a rounding
+// midpoint for these c (a midpoint would need c times a 54-bit odd integer to
+// fit in 53 bits) while the correction is off by at most ulp * 2^-52.
+__device__ __forceinline__ double div_const(double x, double c, double r) {
+    const double q = x * r;
+    const double e = __builtin_fma(-q, c, x);
+    return __builtin_fma(e, r, q);
+}
+
+enum { TAU_CONST = 0, TAU_PHI = 1, TAU_FIELD = 2 };
+
+// Resident wavefronts per SIMD the register allocation targets: two 8-wavefront
+// workgroups per CU fit the LDS budget, i.e. 4 per SIMD (<= 128 VGPRs).  The
+// rare wall + body-force combination would spill at 128 and gets 2.
+template <bool WALL, int FORCE> constexpr int waves_per_eu() { return (WALL && FORCE != 0) ? 2 : 4; }
+
+// Two threads per lattice node: the first NN threads of the workgroup (whole
+// wavefronts) update the node's 15 phi populations, the second NN threads its
+// 15 momentum populations.  Halving the per-thread state doubles the resident
+// wavefronts per CU, and the shared staging / stencil phases are spread over
+// twice the threads.
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+__global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<WALL, FORCE>()))) void step_fp64(StepArgs a) {
+    constexpr int NN = TX * TY * TZ, NT = 2 * NN;
     constexpr int HX2 = TX + 4, HY2 = TY + 4, HZ2 = TZ + 4, H2 = HX2 * HY2 * HZ2;
     constexpr int HX1 = TX + 2, HY1 = TY + 2, HZ1 = TZ + 2, H1 = HX1 * HY1 * HZ1;
-    constexpr bool FFIELD = (FORCE == FORCE_FIELD);
+    static_assert(NN % 64 == 0, "a role must own whole wavefronts");
 
     __shared__ double s_phi[H2];
     __shared__ double s_p[H1], s_u1[H1], s_u2[H1], s_u3[H1];
-    __shared__ double s_A[H1], s_g0[H1], s_g1[H1], s_g2[H1], s_tau[H1];
-    __shared__ double s_fx[FFIELD ? H1 : 1], s_fy[FFIELD ? H1 : 1], s_fz[FFIELD ? H1 : 1];
+    __shared__ double s_A[H1], s_g0[H1], s_g1[H1], s_g2[H1];
+    __shared__ double s_tau[TAUMODE == TAU_CONST ? 1 : H1];
     __shared__ double s_red[2][NT / 64];
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -64,6 +93,8 @@ __global__ __launch_bounds__(TX* TY* TZ) void step_fp64(StepArgs a) {
     const int y0 = ((tile / a.tiles_x) % a.tiles_y) * TY;
     const int z0 = (tile / (a.tiles_x * a.tiles_y)) * TZ;
     const int tid = threadIdx.x;
+    const bool mom = tid >= NN;          // role: momentum populations (uniform per wavefront)
+    const int nid = mom ? tid - NN : tid;
 
     const double* __restrict__ phi_in = a.mac_in;
     const double* __restrict__ p_in = a.mac_in + N;
@@ -71,26 +102,45 @@ __global__ __launch_bounds__(TX* TY* TZ) void step_fp64(StepArgs a) {
     const double* __restrict__ u2_in = a.mac_in + 3 * N;
     const double* __restrict__ u3_in = a.mac_in + 4 * N;
 
+    // ---------------- phase 0: issue the 15 upstream population loads of this
+    // thread's set first, so their latency overlaps staging and stencils
+    const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    const bool inside = (x < nx) && (y < ny) && (z < nz);
+    // upstream coordinate x - c is entry [c + 1] of {x+1, x, x-1} (periodic wrap)
+    const int xc = inside ? x : 0, yc = inside ? y : 0, zc = inside ? z : 0;
+    const int xs[3] = {xc + 1 == nx ? 0 : xc + 1, xc, xc == 0 ? nx - 1 : xc - 1};
+    const int zs[3] = {zc + 1 == nz ? 0 : zc + 1, zc, zc == 0 ? nz - 1 : zc - 1};
+    const int64_t yo[3] = {(int64_t)nx * (yc + 1 == ny ? 0 : yc + 1), (int64_t)nx * yc,
+                           (int64_t)nx * (yc == 0 ? ny - 1 : yc - 1)};
+    const int64_t nxy = (int64_t)nx * ny;
+    const int64_t zo[3] = {nxy * zs[0], nxy * zs[1], nxy * zs[2]};
+    const double* __restrict__ pin = mom ? a.g_in : a.f_in;
+    double ps[15];
+    if (inside) {
+        static_for<0, 15>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            using D = Dir<q>;
+            ps[q] = pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]];
+        });
+    }
+
     // ---------------- phase 1: stage macroscopic fields
+    const bool wm = a.wrap_mod != 0;
     for (int h = tid; h < H2; h += NT) {
         const int hx = h % HX2, hy = (h / HX2) % HY2, hz = h / (HX2 * HY2);
-        const int gx = wrapi(x0 + hx - 2, nx), gy = wrapi(y0 + hy - 2, ny), gz = wrapi(z0 + hz - 2, nz);
+        const int gx = wrapi(x0 + hx - 2, nx, wm), gy = wrapi(y0 + hy - 2, ny, wm), gz = wrapi(z0 + hz - 2, nz, wm);
         s_phi[h] = phi_in[gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
     }
     for (int h = tid; h < H1; h += NT) {
         const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
-        const int gx = wrapi(x0 + hx - 1, nx), gy = wrapi(y0 + hy - 1, ny), gz = wrapi(z0 + hz - 1, nz);
+        const int gx = wrapi(x0 + hx - 1, nx, wm), gy = wrapi(y0 + hy - 1, ny, wm), gz = wrapi(z0 + hz - 1, nz, wm);
         const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
         s_p[h] = p_in[g];
         s_u1[h] = u1_in[g];
         s_u2[h] = u2_in[g];
         s_u3[h] = u3_in[g];
-        if constexpr (TAUFIELD) s_tau[h] = a.tau_field[g];
-        if constexpr (FFIELD) {
-            s_fx[h] = a.fx[g];
-            s_fy[h] = a.fy[g];
-            s_fz[h] = a.fz[g];
-        }
+        if constexpr (TAUMODE == TAU_FIELD) s_tau[h] = a.tau_field[g];
     }
     __syncthreads();
 
@@ -113,16 +163,16 @@ __global__ __launch_bounds__(TX* TY* TZ) void step_fp64(StepArgs a) {
             g1 = acc<D::cy>(g1, v);
             g2 = acc<D::cz>(g2, v);
         });
-        lap = lap / 5.0;
-        g0 = g0 / 10.0;
-        g1 = g1 / 10.0;
-        g2 = g2 / 10.0;
+        lap = div_const(lap, 5.0, 0.2);
+        g0 = div_const(g0, 10.0, 0.1);
+        g1 = div_const(g1, 10.0, 0.1);
+        g2 = div_const(g2, 10.0, 0.1);
         const double p0 = ph * a.T / (1.0 - a.b * ph) - a.a * (ph * ph);
         s_A[h] = p0 - a.kf * ph * lap - a.kf6 * (g0 * g0 + g1 * g1 + g2 * g2);
         s_g0[h] = g0;
         s_g1[h] = g1;
         s_g2[h] = g2;
-        if constexpr (!TAUFIELD) {
+        if constexpr (TAUMODE == TAU_PHI) {
             double frac = (ph - rmin) / range;
             if (range == 0.0) frac = 0.0;
             const double nu = frac * a.dnu + a.nu1;
@@ -131,119 +181,129 @@ __global__ __launch_bounds__(TX* TY* TZ) void step_fp64(StepArgs a) {
     }
     __syncthreads();
 
-    // ---------------- phase 3: collide-stream at each output node
-    const int tx = tid % TX, ty = (tid / TX) % TY, tz = tid / (TX * TY);
-    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
-    const bool inside = (x < nx) && (y < ny) && (z < nz);
-    double phin = 0.0;
+    // ---------------- phase 3: collide-stream of this thread's population set
+    double vmax = -__builtin_huge_val(), vmin = __builtin_huge_val();
     if (inside) {
-        // upstream coordinate x - c is entry [c + 1] of {x+1, x, x-1} (periodic wrap)
-        const int64_t xo[3] = {x + 1 == nx ? 0 : x + 1, x, x == 0 ? nx - 1 : x - 1};
-        const int64_t yo[3] = {(int64_t)nx * (y + 1 == ny ? 0 : y + 1), (int64_t)nx * y,
-                               (int64_t)nx * (y == 0 ? ny - 1 : y - 1)};
-        const int64_t nxy = (int64_t)nx * ny;
-        const int64_t zo[3] = {nxy * (z + 1 == nz ? 0 : z + 1), nxy * z, nxy * (z == 0 ? nz - 1 : z - 1)};
-        const int xs[3] = {(int)xo[0], (int)xo[1], (int)xo[2]};
-        const int zs[3] = {z + 1 == nz ? 0 : z + 1, z, z == 0 ? nz - 1 : z - 1};
-
-        double fq[15], gq[15];
-        static_for<0, 15>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            using D = Dir<q>;
-            const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
-            const int h2 = (tx + 2 - D::cx) + HX2 * ((ty + 2 - D::cy) + HY2 * (tz + 2 - D::cz));
-            NodeEq<double> nd;
-            nd.phi = s_phi[h2];
-            nd.p = s_p[hu];
-            nd.u1 = s_u1[hu];
-            nd.u2 = s_u2[hu];
-            nd.u3 = s_u3[hu];
-            nd.A = s_A[hu];
-            nd.g0 = s_g0[hu];
-            nd.g1 = s_g1[hu];
-            nd.g2 = s_g2[hu];
-            nd.tau = s_tau[hu];
-            const double usq = nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
-            const double s2h = 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
-            const double gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
-            const double fe = feq<q>(nd, gt, a.kf);
-            const double ge = geq<q>(nd, usq, gt, a.kg);
-            const int64_t src = xo[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
-            const double fs = a.f_in[q * N + src];
-            const double gs = a.g_in[q * N + src];
-            fq[q] = fs - (fs - fe) / a.tau_f;
-            double gv = gs - (gs - ge) / nd.tau;
-            if constexpr (FORCE != FORCE_NONE && q != 0) {
-                double Fx, Fy, Fz;
-                if constexpr (FORCE == FORCE_TAYLOR_GREEN) {
-                    const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
-                    Fx = a.tg_amp * a.sinx[xu] * a.cosz[zu];
-                    Fy = 0.0;
-                    Fz = -(a.tg_amp * a.cosx[xu] * a.sinz[zu]);
-                } else {
-                    Fx = s_fx[hu];
-                    Fy = s_fy[hu];
-                    Fz = s_fz[hu];
-                }
-                double dot = (double)D::cx * Fx + (double)D::cy * Fy;
-                dot = dot + (double)D::cz * Fz;
-                gv = gv + 3.0 * weightE<q>() * dot;
-            }
-            gq[q] = gv;
-        });
-
-        if constexpr (WALL) {
-            // bounce_back_LBM, dropch.f90:666-698: on the first/last fluid layer
-            // the populations entering the fluid are the reflected ones minus
-            // 6 E (U_wall . c_opp); bottom wall moves with -U, top with +U.
-            const int coord = (a.wall_axis == 2) ? y : z;
-            const int nax = (a.wall_axis == 2) ? ny : nz;
-            const bool bottom = coord == 0, top = coord == nax - 1;
-            if (bottom || top) {
-                const double uw = bottom ? -a.wall_u_now : a.wall_u_now;
-                const int want = bottom ? 1 : -1;
-                static_for<1, 15>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    using D = Dir<q>;
-                    constexpr int o = D::opp;
-                    const int ca = (a.wall_axis == 2) ? D::cy : D::cz;
-                    if (ca == want) {
-                        const double dot = uw * (double)Dir<o>::cx;
-                        gq[q] = gq[o] - 6.0 * weightE<o>() * dot;
-                    }
-                });
-            }
-        }
-
-        // moments (physics :546-552), reference order i = 1..15
-        double ph = fq[0], pr = gq[0] / 3.0;
-        double m1 = 0.0, m2 = 0.0, m3 = 0.0;
-        static_for<1, 15>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            using D = Dir<q>;
-            ph = ph + fq[q];
-            pr = pr + gq[q] / 3.0;
-            m1 = acc<D::cx>(m1, gq[q]);
-            m2 = acc<D::cy>(m2, gq[q]);
-            m3 = acc<D::cz>(m3, gq[q]);
-        });
         const int64_t s = x + (int64_t)nx * (y + (int64_t)ny * z);
-        static_for<0, 15>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            a.f_out[q * N + s] = fq[q];
-            a.g_out[q * N + s] = gq[q];
-        });
-        a.mac_out[s] = ph;
-        a.mac_out[N + s] = pr;
-        a.mac_out[2 * N + s] = m1;
-        a.mac_out[3 * N + s] = m2;
-        a.mac_out[4 * N + s] = m3;
-        phin = ph;
+        if (!mom) {
+            double ph = 0.0;
+            static_for<0, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                const int h2 = (tx + 2 - D::cx) + HX2 * ((ty + 2 - D::cy) + HY2 * (tz + 2 - D::cz));
+                NodeEq<double> nd;
+                nd.phi = s_phi[h2];
+                nd.A = s_A[hu];
+                double gt = 0.0;
+                if constexpr (q != 0) {
+                    nd.u1 = s_u1[hu];
+                    nd.u2 = s_u2[hu];
+                    nd.u3 = s_u3[hu];
+                    nd.g0 = s_g0[hu];
+                    nd.g1 = s_g1[hu];
+                    nd.g2 = s_g2[hu];
+                    const double s2h = 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
+                    gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
+                }
+                const double fe = feq<q>(nd, gt, a.kf);
+                const double fv = ps[q] - (ps[q] - fe) / a.tau_f;
+                ph = (q == 0) ? fv : ph + fv;
+                __builtin_nontemporal_store(fv, &a.f_out[q * N + s]);
+                __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
+            });
+            __builtin_nontemporal_store(ph, &a.mac_out[s]);
+            vmax = ph;
+            vmin = ph;
+        } else {
+            static_for<0, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                NodeEq<double> nd;
+                nd.p = s_p[hu];
+                nd.u1 = s_u1[hu];
+                nd.u2 = s_u2[hu];
+                nd.u3 = s_u3[hu];
+                const double usq = nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
+                double gt = 0.0;
+                if constexpr (q != 0) {
+                    nd.g0 = s_g0[hu];
+                    nd.g1 = s_g1[hu];
+                    nd.g2 = s_g2[hu];
+                    const double s2h = 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
+                    gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
+                }
+                const double ge = geq<q>(nd, usq, gt, a.kg);
+                const double tau = (TAUMODE == TAU_CONST) ? a.tau_const : s_tau[hu];
+                double gv = ps[q] - (ps[q] - ge) / tau;
+                if constexpr (FORCE != FORCE_NONE && q != 0) {
+                    double Fx, Fy, Fz;
+                    if constexpr (FORCE == FORCE_TAYLOR_GREEN) {
+                        const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
+                        Fx = a.tg_amp * a.sinx[xu] * a.cosz[zu];
+                        Fy = 0.0;
+                        Fz = -(a.tg_amp * a.cosx[xu] * a.sinz[zu]);
+                    } else {  // static field, read at the upstream node through L2
+                        const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
+                        Fx = a.fx[src];
+                        Fy = a.fy[src];
+                        Fz = a.fz[src];
+                    }
+                    double dot = (double)D::cx * Fx + (double)D::cy * Fy;
+                    dot = dot + (double)D::cz * Fz;
+                    gv = gv + 3.0 * weightE<q>() * dot;
+                }
+                ps[q] = gv;
+                __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
+            });
+
+            if constexpr (WALL) {
+                // bounce_back_LBM, dropch.f90:666-698: on the first/last fluid layer
+                // the populations entering the fluid are the reflected ones minus
+                // 6 E (U_wall . c_opp); bottom wall moves with -U, top with +U.
+                const int coord = (a.wall_axis == 2) ? y : z;
+                const int nax = (a.wall_axis == 2) ? ny : nz;
+                const bool bottom = coord == 0, top = coord == nax - 1;
+                if (bottom || top) {
+                    const double uw = bottom ? -a.wall_u_now : a.wall_u_now;
+                    const int want = bottom ? 1 : -1;
+                    static_for<1, 15>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        using D = Dir<q>;
+                        constexpr int o = D::opp;
+                        const int ca = (a.wall_axis == 2) ? D::cy : D::cz;
+                        if (ca == want) {
+                            const double dot = uw * (double)Dir<o>::cx;
+                            ps[q] = ps[o] - 6.0 * weightE<o>() * dot;
+                        }
+                    });
+                }
+            }
+
+            // moments p = sum g/3, u = sum c g (physics :546-552), order i = 1..15
+            double pr = div_const(ps[0], 3.0, 1.0 / 3.0);
+            double m1 = 0.0, m2 = 0.0, m3 = 0.0;
+            static_for<1, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                pr = pr + div_const(ps[q], 3.0, 1.0 / 3.0);
+                m1 = acc<D::cx>(m1, ps[q]);
+                m2 = acc<D::cy>(m2, ps[q]);
+                m3 = acc<D::cz>(m3, ps[q]);
+            });
+            static_for<0, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                __builtin_nontemporal_store(ps[q], &a.g_out[q * N + s]);
+            });
+            __builtin_nontemporal_store(pr, &a.mac_out[N + s]);
+            __builtin_nontemporal_store(m1, &a.mac_out[2 * N + s]);
+            __builtin_nontemporal_store(m2, &a.mac_out[3 * N + s]);
+            __builtin_nontemporal_store(m3, &a.mac_out[4 * N + s]);
+        }
     }
 
     // ---------------- global phi range for the next step's nu(phi)
-    double vmax = inside ? phin : -__builtin_huge_val();
-    double vmin = inside ? phin : __builtin_huge_val();
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
@@ -257,7 +317,7 @@ __global__ __launch_bounds__(TX* TY* TZ) void step_fp64(StepArgs a) {
     __syncthreads();
     if (tid == 0) {
         double mx = s_red[0][0], mn = s_red[1][0];
-        for (int w = 1; w < NT / 64; w++) {
+        for (int w = 1; w < NN / 64; w++) {
             mx = fmax(mx, s_red[0][w]);
             mn = fmin(mn, s_red[1][w]);
         }
@@ -273,37 +333,39 @@ __global__ __launch_bounds__(TX* TY* TZ) void step_fp64(StepArgs a) {
 }
 
 // ------------------------------------------------------------------ launcher
-template <int TX, int TY, int TZ, bool WALL, int FORCE, bool TAUF>
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUM>
 static hipError_t launch_shape(const StepArgs& a, hipStream_t st) {
     StepArgs b = a;
     b.tiles_x = (a.nx + TX - 1) / TX;
     b.tiles_y = (a.ny + TY - 1) / TY;
     b.tiles_z = (a.nz + TZ - 1) / TZ;
+    b.wrap_mod = (a.nx <= TX || a.ny <= TY || a.nz <= TZ) ? 1 : 0;
     const int nb = b.tiles_x * b.tiles_y * b.tiles_z;
-    hipLaunchKernelGGL((step_fp64<TX, TY, TZ, WALL, FORCE, TAUF>), dim3(nb), dim3(TX * TY * TZ), 0, st, b);
+    hipLaunchKernelGGL((step_fp64<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(nb), dim3(2 * TX * TY * TZ), 0, st, b);
     return hipGetLastError();
+}
+
+template <int TX, int TY, int TZ, bool WALL, int FORCE>
+static hipError_t launch_tau(const StepArgs& a, hipStream_t st) {
+    if (a.tau_field) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_FIELD>(a, st);
+    if (a.dnu == 0.0) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_CONST>(a, st);
+    return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_PHI>(a, st);
 }
 
 template <int TX, int TY, int TZ>
 static hipError_t launch_variant(const StepArgs& a, hipStream_t st) {
     const bool wall = a.wall_axis != 0;
-    const bool tauf = a.tau_field != nullptr;
-#define DRP_CASE(W, F, T)                                                             \
-    if (wall == W && a.force_kind == F && tauf == T) return launch_shape<TX, TY, TZ, W, F, T>(a, st);
-    DRP_CASE(false, FORCE_NONE, false)
-    DRP_CASE(true, FORCE_NONE, false)
-    DRP_CASE(false, FORCE_TAYLOR_GREEN, false)
-    DRP_CASE(true, FORCE_TAYLOR_GREEN, false)
-    DRP_CASE(false, FORCE_FIELD, false)
-    DRP_CASE(true, FORCE_FIELD, false)
-    DRP_CASE(false, FORCE_NONE, true)
-    DRP_CASE(true, FORCE_NONE, true)
-    DRP_CASE(false, FORCE_TAYLOR_GREEN, true)
-    DRP_CASE(true, FORCE_TAYLOR_GREEN, true)
-    DRP_CASE(false, FORCE_FIELD, true)
-    DRP_CASE(true, FORCE_FIELD, true)
-#undef DRP_CASE
-    return hipErrorInvalidValue;
+    switch (a.force_kind) {
+        case FORCE_NONE:
+            return wall ? launch_tau<TX, TY, TZ, true, FORCE_NONE>(a, st) : launch_tau<TX, TY, TZ, false, FORCE_NONE>(a, st);
+        case FORCE_TAYLOR_GREEN:
+            return wall ? launch_tau<TX, TY, TZ, true, FORCE_TAYLOR_GREEN>(a, st)
+                        : launch_tau<TX, TY, TZ, false, FORCE_TAYLOR_GREEN>(a, st);
+        case FORCE_FIELD:
+            return wall ? launch_tau<TX, TY, TZ, true, FORCE_FIELD>(a, st) : launch_tau<TX, TY, TZ, false, FORCE_FIELD>(a, st);
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st) {
@@ -312,6 +374,9 @@ hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st) {
         case SHAPE_32x4x2: return launch_variant<32, 4, 2>(a, st);
         case SHAPE_64x2x2: return launch_variant<64, 2, 2>(a, st);
         case SHAPE_32x2x4: return launch_variant<32, 2, 4>(a, st);
+        case SHAPE_32x4x4: return launch_variant<32, 4, 4>(a, st);
+        case SHAPE_32x8x2: return launch_variant<32, 8, 2>(a, st);
+        case SHAPE_64x4x2: return launch_variant<64, 4, 2>(a, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -61,6 +61,7 @@ def lib():
         L.drp_set_tile_shape.argtypes = [vp, C.c_int]
         L.drp_get_tile_shape.argtypes = [vp]
         L.drp_autotune.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_int), dp]
+        L.drp_copy_bandwidth.argtypes = [C.c_size_t, C.c_int, dp]
         _lib = L
     return _lib
 
@@ -191,3 +192,10 @@ def autotune(cfg: Config, nsteps: int = 10):
     ms = (C.c_double * n)()
     _check(lib().drp_autotune(C.byref(cfg), int(nsteps), C.byref(best), ms), "drp_autotune")
     return best.value, list(ms)
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """GB/s of a plain device stream copy (read + write counted)."""
+    gbs = C.c_double()
+    _check(lib().drp_copy_bandwidth(int(nbytes), int(reps), C.byref(gbs)), "drp_copy_bandwidth")
+    return gbs.value

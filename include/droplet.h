@@ -34,6 +34,8 @@
 extern "C" {
 #endif
 
+#define DROPLET_ABI_VERSION 1
+
 #define DRP_OK 0
 #define DRP_ERR_ARG (-1)         /* invalid argument / size mismatch */
 #define DRP_ERR_HIP (-2)         /* HIP runtime failure              */
@@ -97,14 +99,18 @@ typedef struct drp_stats {
     double phi_mass;      /* sum phi                                                   */
     double phi_min, phi_max; /* range used by nu(phi) at the next step (ref physics :559-574) */
     double kinetic;       /* 0.5 <|u|^2>  (mean over nodes)                            */
-    double dissipation;   /* nu1 <2 S:S> (mean over nodes)                             */
+    double dissipation;   /* 0.5 nu1 <(d_a u_b + d_b u_a)^2>, mean over nodes          */
     double interface_area;/* sum |grad phi| (ref drop_analysis.f90:445-474)            */
-    double free_energy;   /* sum psi(phi) + kappa_f/2 |grad phi|^2                     */
+    double free_energy;   /* sum phi T ln(phi/(1-b phi)) - a phi^2 (bulk part)         */
     double p_mean;        /* <p>                                                       */
     double momentum[3];   /* sum u                                                     */
 } drp_stats;
 
 typedef struct drp_lattice drp_lattice; /* opaque; owns all device memory */
+
+/* Layout check for bindings (Fortran, ctypes): sizes of the three structs as
+ * this library was compiled; returns DROPLET_ABI_VERSION.  Needs no GPU. */
+int drp_abi_sizes(size_t* config, size_t* drop, size_t* stats);
 
 /* Fill a config with the reference defaults of program/lbm_drop_xy.f90:9-67
  * (phi1=2.211, phi2=4.895, a=9/49, b=2/21, T=0.55, kappa_f=0.01, tau_f=0.7). */
@@ -173,6 +179,10 @@ int drp_get_tile_shape(const drp_lattice* lat);
  * droplet, nsteps steps each after 2 warm-up steps) and return the fastest;
  * ms_per_step (may be NULL) receives drp_tile_shape_count() timings. */
 int drp_autotune(const drp_config* cfg, int nsteps, int* best_shape, double* ms_per_step);
+
+/* Achievable HBM bandwidth of a plain stream copy of `bytes` (read + write counted),
+ * averaged over reps launches on a private stream. */
+int drp_copy_bandwidth(size_t bytes, int reps, double* gbs);
 
 /* Message for the calling thread's last failure ("" if none). */
 const char* drp_last_error(void);

@@ -17,6 +17,11 @@
  * periodic index wrap replaces the reference's 1-cell halos (glue/glue_f,
  * program/lbm_drop_xy.f90:287-384).
  *
+ * ORACLE_PLANT (compile-time, default 0) plants one known mistake so the tests
+ * can show each pin catches it: 1 drops the Korteweg term of geq, 2 flips the
+ * sign of the body force, 3 divides the Laplacian by 10 instead of 5, 4 moves
+ * the bottom wall with +U instead of -U.  Only tests build planted variants.
+ *
  * Parity pins (tests/test_oracle_*.py): golden vectors produced by the compiled
  * reference (tests/golden/), lattice identities, moment identities, polynomial
  * stencils, mass conservation, Couette flow between moving walls, Taylor-Green
@@ -28,6 +33,10 @@
 #include <string.h>
 
 #include "../include/droplet.h"
+
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
 
 /* D3Q15 table, program/lbm_drop_xy.f90:70-84 */
 static const int CX[15] = {0, 1, 0, 0, -1, 0, 0, 1, -1, 1, 1, -1, 1, -1, -1};
@@ -108,7 +117,7 @@ static void stencils_at(ora_lattice* L, int x, int y, int z) {
     const double* phi = L->phi;
     double lap = -14.0 * phi[s];                    /* :608 */
     for (int i = 1; i < 15; i++) lap = lap + phi[nb[i]];
-    L->lap[s] = lap / 5.0;                          /* :613, 5 ds^2 */
+    L->lap[s] = lap / (ORACLE_PLANT == 3 ? 10.0 : 5.0); /* :613, 5 ds^2 */
     for (int a = 0; a < 3; a++) {                   /* :616-633 */
         const int* ca = a == 0 ? CX : (a == 1 ? CY : CZ);
         double gphi = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
@@ -196,7 +205,7 @@ static void equilibria_at(ora_lattice* L, int64_t s) {
                                + EW[i] * c->kappa_f * gt;                           /* :452-459 */
         double cu = u1 * (double)CX[i] + u2 * (double)CY[i] + u3 * (double)CZ[i];
         L->geq[i * L->n + s] = EW[i] * (3.0 * p + 3.0 * cu + 4.5 * (cu * cu) - 1.5 * usq + c->anu * ft)
-                               + EW[i] * c->kappa_g * gt;                           /* :461-469 */
+                               + (ORACLE_PLANT == 1 ? 0.0 : EW[i] * c->kappa_g * gt); /* :461-469 */
     }
 }
 
@@ -323,8 +332,8 @@ static void stream_collide(ora_lattice* L, int force_on) {
                     L->fn[i * n + s] = fs - (fs - L->feq[i * n + u]) / c->tau_f;
                     double gv = gs - (gs - L->geq[i * n + u]) / L->tau[u];
                     if (force_on)
-                        gv = gv + 3.0 * EW[i] * ((double)CX[i] * L->fx[u] + (double)CY[i] * L->fy[u]
-                                                 + (double)CZ[i] * L->fz[u]);
+                        gv = gv + (ORACLE_PLANT == 2 ? -3.0 : 3.0) * EW[i]
+                                      * ((double)CX[i] * L->fx[u] + (double)CY[i] * L->fy[u] + (double)CZ[i] * L->fz[u]);
                     L->gn[i * n + s] = gv;
                 }
             }
@@ -339,7 +348,7 @@ static void bounce_back(ora_lattice* L) {
     const int n_axis = (c->wall_axis == DRP_WALL_Y) ? c->ny : c->nz;
     for (int side = 0; side < 2; side++) {
         int layer = side == 0 ? 0 : n_axis - 1;
-        double uw = side == 0 ? -U : U;  /* bottom wall moves with -U, top with +U */
+        double uw = side == 0 ? (ORACLE_PLANT == 4 ? U : -U) : U; /* bottom wall -U, top +U */
         for (int i = 0; i < 15; i++) {
             int ca = (c->wall_axis == DRP_WALL_Y) ? CY[i] : CZ[i];
             if (ca != (side == 0 ? 1 : -1)) continue; /* populations entering the fluid */
@@ -611,4 +620,28 @@ int ora_deformation(const ora_lattice* L, int mode, double* Lout, double* Bout, 
     if (mode == 0) return deformation_breakup3(L, Lout, Bout, Dout);
     if (mode == 1) return deformation_area(L, Lout, Bout, Dout);
     return DRP_ERR_ARG;
+}
+
+/* ---------------------------------------------------------------- inspection */
+
+/* Stencil quantities of the current phi and u (various_cal :590-640):
+   lap [n], grad phi [3][n], grad u [9][n] with gu[k*3+a] = d_a u_k. */
+int ora_stencils(ora_lattice* L, double* lap, double* gphi, double* gu) {
+    const drp_config* c = &L->c;
+    for (int z = 0; z < c->nz; z++)
+        for (int y = 0; y < c->ny; y++)
+            for (int x = 0; x < c->nx; x++) stencils_at(L, x, y, z);
+    memcpy(lap, L->lap, sizeof(double) * (size_t)L->n);
+    memcpy(gphi, L->gp, sizeof(double) * 3 * (size_t)L->n);
+    memcpy(gu, L->gu, sizeof(double) * 9 * (size_t)L->n);
+    return 0;
+}
+
+/* Equilibria of the current fields and range (equilibrium_cal :421-476) and tau_g. */
+int ora_equilibria(ora_lattice* L, double* feq, double* geq, double* tau) {
+    compute_equilibria(L);
+    memcpy(feq, L->feq, sizeof(double) * 15 * (size_t)L->n);
+    memcpy(geq, L->geq, sizeof(double) * 15 * (size_t)L->n);
+    memcpy(tau, L->tau, sizeof(double) * (size_t)L->n);
+    return 0;
 }

@@ -23,32 +23,48 @@ def build() -> str:
     return LIB_PATH
 
 
-def lib():
+def build_planted(plant: int, out_dir: str) -> str:
+    """Compile a copy of the restatement with one planted mistake (see ORACLE_PLANT)."""
+    out = os.path.join(out_dir, f"liblbm_oracle_plant{plant}.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c11", f"-DORACLE_PLANT={plant}",
+                    "-shared", "-o", out, os.path.join(HERE, "lbm_oracle.c"), "-lm"], check=True)
+    return out
+
+
+def lib(path: str | None = None):
+    """ctypes binding of the restatement (the default build, or the library at `path`)."""
     global _lib
+    if path is not None:
+        return _bind(C.CDLL(path))
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        vp, dp = C.c_void_p, C.POINTER(C.c_double)
-        L.ora_create.restype = vp
-        L.ora_create.argtypes = [C.POINTER(Config)]
-        L.ora_destroy.argtypes = [vp]
-        L.ora_set_threads.argtypes = [vp, C.c_int]
-        L.ora_step_count.restype = C.c_int64
-        L.ora_step_count.argtypes = [vp]
-        L.ora_init_droplets.argtypes = [vp, C.POINTER(Drop), C.c_int, C.c_double]
-        L.ora_init_fields.argtypes = [vp, dp, dp, dp, dp, dp]
-        L.ora_step.argtypes = [vp, C.c_int]
-        L.ora_get_macro.argtypes = [vp, C.c_int, dp]
-        L.ora_get_populations.argtypes = [vp, C.c_int, dp]
-        L.ora_set_populations.argtypes = [vp, C.c_int, dp]
-        L.ora_inject_momentum.argtypes = [vp, dp]
-        L.ora_set_force.argtypes = [vp, dp, dp, dp]
-        L.ora_phi_range.argtypes = [vp, dp, dp]
-        L.ora_reduce.argtypes = [vp, C.POINTER(Stats)]
-        L.ora_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
-        _lib = L
+        _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
+
+
+def _bind(L):
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    L.ora_create.restype = vp
+    L.ora_create.argtypes = [C.POINTER(Config)]
+    L.ora_destroy.argtypes = [vp]
+    L.ora_set_threads.argtypes = [vp, C.c_int]
+    L.ora_step_count.restype = C.c_int64
+    L.ora_step_count.argtypes = [vp]
+    L.ora_init_droplets.argtypes = [vp, C.POINTER(Drop), C.c_int, C.c_double]
+    L.ora_init_fields.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.ora_step.argtypes = [vp, C.c_int]
+    L.ora_get_macro.argtypes = [vp, C.c_int, dp]
+    L.ora_get_populations.argtypes = [vp, C.c_int, dp]
+    L.ora_set_populations.argtypes = [vp, C.c_int, dp]
+    L.ora_inject_momentum.argtypes = [vp, dp]
+    L.ora_set_force.argtypes = [vp, dp, dp, dp]
+    L.ora_phi_range.argtypes = [vp, dp, dp]
+    L.ora_reduce.argtypes = [vp, C.POINTER(Stats)]
+    L.ora_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.ora_stencils.argtypes = [vp, dp, dp, dp]
+    L.ora_equilibria.argtypes = [vp, dp, dp, dp]
+    return L
 
 
 def _dp(a: np.ndarray):
@@ -59,41 +75,42 @@ def _dp(a: np.ndarray):
 class OracleLattice:
     """Same call surface as droplet_amd.Lattice, computed by the CPU restatement."""
 
-    def __init__(self, cfg: Config, threads: int | None = None):
+    def __init__(self, cfg: Config, threads: int | None = None, libpath: str | None = None):
+        self._L = lib(libpath) if libpath else lib()
         self.cfg = cfg
         self.shape = (cfg.nz, cfg.ny, cfg.nx)
         self.n = cfg.nx * cfg.ny * cfg.nz
-        self._h = lib().ora_create(C.byref(cfg))
+        self._h = self._L.ora_create(C.byref(cfg))
         if not self._h:
             raise ValueError("ora_create failed")
         if threads is None:
             threads = int(os.environ.get("DROPLET_ORACLE_THREADS", min(8, os.cpu_count() or 1)))
-        lib().ora_set_threads(self._h, threads)
+        self._L.ora_set_threads(self._h, threads)
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().ora_destroy(h)
+            self._L.ora_destroy(h)
             self._h = None
 
     def init_droplets(self, drops, p_init: float = 1.0 / 3.0):
         arr = (Drop * len(drops))(*[Drop(*d) for d in drops])
-        lib().ora_init_droplets(self._h, arr, len(drops), p_init)
+        self._L.ora_init_droplets(self._h, arr, len(drops), p_init)
 
     def init_fields(self, phi, p, u1, u2, u3):
         fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (phi, p, u1, u2, u3)]
-        lib().ora_init_fields(self._h, *[_dp(a) for a in fs])
+        self._L.ora_init_fields(self._h, *[_dp(a) for a in fs])
 
     def step(self, n: int = 1):
-        lib().ora_step(self._h, int(n))
+        self._L.ora_step(self._h, int(n))
 
     @property
     def step_count(self) -> int:
-        return int(lib().ora_step_count(self._h))
+        return int(self._L.ora_step_count(self._h))
 
     def macro(self, field: int) -> np.ndarray:
         out = np.empty(self.shape, np.float64)
-        if lib().ora_get_macro(self._h, field, _dp(out)) != 0:
+        if self._L.ora_get_macro(self._h, field, _dp(out)) != 0:
             raise ValueError(field)
         return out
 
@@ -102,36 +119,52 @@ class OracleLattice:
 
     def populations(self, which: int) -> np.ndarray:
         out = np.empty((15,) + self.shape, np.float64)
-        lib().ora_get_populations(self._h, which, _dp(out))
+        self._L.ora_get_populations(self._h, which, _dp(out))
         return out
 
     def set_populations(self, which: int, pops: np.ndarray):
         a = np.ascontiguousarray(pops, np.float64)
         assert a.shape == (15,) + self.shape
-        lib().ora_set_populations(self._h, which, _dp(a))
+        self._L.ora_set_populations(self._h, which, _dp(a))
 
     def inject_momentum(self, g: np.ndarray):
         a = np.ascontiguousarray(g, np.float64)
         assert a.shape == (15,) + self.shape
-        lib().ora_inject_momentum(self._h, _dp(a))
+        self._L.ora_inject_momentum(self._h, _dp(a))
 
     def set_force(self, fx, fy, fz):
         fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (fx, fy, fz)]
-        lib().ora_set_force(self._h, *[_dp(a) for a in fs])
+        self._L.ora_set_force(self._h, *[_dp(a) for a in fs])
 
     def phi_range(self):
         mn, mx = C.c_double(), C.c_double()
-        lib().ora_phi_range(self._h, C.byref(mn), C.byref(mx))
+        self._L.ora_phi_range(self._h, C.byref(mn), C.byref(mx))
         return mn.value, mx.value
 
     def reduce(self) -> dict:
         st = Stats()
-        lib().ora_reduce(self._h, C.byref(st))
+        self._L.ora_reduce(self._h, C.byref(st))
         return st.as_dict()
 
     def deformation(self, mode: int = 0):
         L, B, D = C.c_double(), C.c_double(), C.c_double()
-        rc = lib().ora_deformation(self._h, mode, C.byref(L), C.byref(B), C.byref(D))
+        rc = self._L.ora_deformation(self._h, mode, C.byref(L), C.byref(B), C.byref(D))
         if rc != 0:
             raise ValueError(f"deformation failed ({rc})")
         return L.value, B.value, D.value
+
+    def stencils(self):
+        """(lap phi, grad phi [3], grad u [3][3] with [k][a] = d_a u_k) of the current fields."""
+        lap = np.empty(self.shape)
+        gp = np.empty((3,) + self.shape)
+        gu = np.empty((9,) + self.shape)
+        self._L.ora_stencils(self._h, _dp(lap), _dp(gp), _dp(gu))
+        return lap, gp, gu.reshape((3, 3) + self.shape)
+
+    def equilibria(self):
+        """(feq [15], geq [15], tau_g) of the current fields."""
+        fe = np.empty((15,) + self.shape)
+        ge = np.empty((15,) + self.shape)
+        tau = np.empty(self.shape)
+        self._L.ora_equilibria(self._h, _dp(fe), _dp(ge), _dp(tau))
+        return fe, ge, tau
