@@ -56,9 +56,11 @@ def traffic_from_profiles(kernel_tag: str):
         return None
 
 
-def run_gpu(cfg, drops, steps, warmup):
+def run_gpu(cfg, drops, steps, warmup, shape=None):
     from droplet_amd.lattice import Lattice
     lat = Lattice(cfg)
+    if shape is not None:
+        lat.tile_shape = shape
     lat.init_droplets(drops, 1.0 / 3.0)
     if warmup:
         lat.step(warmup)
@@ -116,6 +118,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: the whole domain runs on a single MI355X "
@@ -131,6 +134,33 @@ def main():
     achieved = BYTES_PER_LU["f64"] * nodes / (kernel_ms * 1e-3) / 1e9
     stats = lat.reduce()
     tile = lat.tile_shape
+
+    # FP32 shifted storage: same workload, same step count from the same initial
+    # state; its error is stated against the FP64 run above
+    fp32 = None
+    if not args.no_fp32:
+        import numpy as np
+        cfg32, _, _ = shear_config(N, precision=1)
+        lat32, wall32, ms32 = run_gpu(cfg32, drops, args.steps, args.warmup)
+        k32 = ms32 / args.steps
+        ach32 = BYTES_PER_LU["f32"] * nodes / (k32 * 1e-3) / 1e9
+        phi64, phi32 = lat.macro(0), lat32.macro(0)
+        u64, u32 = lat.macro(2), lat32.macro(2)
+        D64, D32 = lat.deformation(2)[2], lat32.deformation(2)[2]
+        m32 = lat32.reduce()["phi_mass"]
+        mlups32 = nodes * args.steps / wall32 / 1e6
+        fp32 = {"value": round(mlups32, 2), "unit": "MLUPS", "dtype": "f32 (shifted storage)",
+                "ms_per_step": round(wall32 * 1e3 / args.steps, 4), "speedup_vs_fp64": round(mlups32 / mlups, 3),
+                "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"step_fp32_{N}"),
+                             "kernel_ms": round(k32, 5), "bytes_per_lu": BYTES_PER_LU["f32"]},
+                "error_vs_fp64": {"after_steps": args.warmup + args.steps,
+                                  "phi_linf_rel": float(np.abs(phi32 - phi64).max() / cfg.phi2),
+                                  "u1_linf": float(np.abs(u32 - u64).max()),
+                                  "deformation_abs": abs(D32 - D64), "deformation_fp64": D64,
+                                  "deformation_measure": "second moments of the shear plane (mode 2)",
+                                  "phi_mass_rel": abs(m32 - stats["phi_mass"]) / stats["phi_mass"]}}
+        del lat32
     del lat
 
     cpu = None
@@ -167,6 +197,7 @@ def main():
                      "traffic": traffic_from_profiles(f"step_fp64_{N}"),
                      "kernel_ms": round(kernel_ms, 5), "bytes_per_lu": BYTES_PER_LU["f64"]},
         "cpu_baseline": cpu,
+        "fp32_shifted": fp32,
         "phi_mass": stats["phi_mass"],
     }
     print(json.dumps(out))

@@ -167,6 +167,47 @@ static int area_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
     return DRP_OK;
 }
 
+// Second-moment variant: weights w = max(phi - phi1, 0) / (phi2 - phi1) in the
+// search plane, centroid-relative second moments I; for a uniform ellipse with
+// semi-axes a >= b the eigenvalues satisfy l+ / l- = (a / b)^2 and l+ = W a^2 / 4,
+// so L = 4 sqrt(l+ / W), B = 4 sqrt(l- / W).  Continuous in phi, unlike the
+// farthest-point and threshold-crossing steps of break_up3.
+static int inertia_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
+    const drp_config& c = L->cfg;
+    const int n3 = (c.wall_axis == DRP_WALL_Y) ? c.nz : c.ny;
+    Plane S;
+    int rc = load(L, n3 / 2, S);
+    if (rc) return rc;
+    const double inv = 1.0 / (c.phi2 - c.phi1);
+    double W = 0.0, sx = 0.0, sz = 0.0;
+    for (int j = 0; j < S.n2; j++)
+        for (int i = 0; i < S.n1; i++) {
+            const double w = std::fmax(S.at(i, j) - c.phi1, 0.0) * inv;
+            W += w;
+            sx += w * i;
+            sz += w * j;
+        }
+    if (W <= 0.0) return drp_fail(DRP_ERR_STATE, "no dispersed phase in the deformation plane");
+    const double xc = sx / W, zc = sz / W;
+    double ixx = 0.0, izz = 0.0, ixz = 0.0;
+    for (int j = 0; j < S.n2; j++)
+        for (int i = 0; i < S.n1; i++) {
+            const double w = std::fmax(S.at(i, j) - c.phi1, 0.0) * inv;
+            const double dx = i - xc, dz = j - zc;
+            ixx += w * dx * dx;
+            izz += w * dz * dz;
+            ixz += w * dx * dz;
+        }
+    const double m = 0.5 * (ixx + izz), r = std::sqrt(0.25 * (ixx - izz) * (ixx - izz) + ixz * ixz);
+    const double lp = m + r, lm = std::fmax(m - r, 0.0);
+    *Lo = 4.0 * std::sqrt(lp / W);
+    *Bo = 4.0 * std::sqrt(lm / W);
+    *Do = (*Lo - *Bo) / (*Lo + *Bo);
+    return DRP_OK;
+}
+
 int deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do) {
-    return mode == 0 ? breakup3(L, Lo, Bo, Do) : area_variant(L, Lo, Bo, Do);
+    if (mode == 0) return breakup3(L, Lo, Bo, Do);
+    if (mode == 1) return area_variant(L, Lo, Bo, Do);
+    return inertia_variant(L, Lo, Bo, Do);
 }

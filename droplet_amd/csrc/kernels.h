@@ -49,6 +49,49 @@ struct StepArgs {
 
 hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st);
 
+// FP32 shifted storage (step_fp32.hip): populations and fields as FP32
+// deviations from the continuous phase at rest
+struct StepArgs32 {
+    int nx, ny, nz;
+    int tiles_x, tiles_y, tiles_z;
+    int64_t n;
+    const float* f_in;
+    const float* g_in;
+    float* f_out;
+    float* g_out;
+    const float* mac_in;   // phi - phi1, p - 1/3, u1, u2, u3
+    float* mac_out;
+    const float* tau_field;
+    const double *fx, *fy, *fz;
+    const double *sinx, *cosx, *sinz, *cosz;
+    double tg_amp, phi1, phi2;
+    float phi1f, Tf, af, bf, one_m_bphi1, kff, kf6f, kgf;
+    float rtau_f, rtau_const, nu1f, dnuf;
+    float wall_u_now;
+    int wall_axis, force_kind;
+    uint64_t* range;
+    int slot_read, slot_write, slot_reset;
+    int wrap_mod;
+};
+hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st);
+
+// conversions between FP64 absolute and FP32 shifted representations; fref is
+// the 15 reference phi populations H_q phi1 + F_q p0(phi1) (host-computed)
+struct ShiftRef {
+    double fr[15];
+    double phi1;
+};
+hipError_t launch_to_shifted(int64_t n, const ShiftRef& r, const double* pop64, const double* mac64, float* pop32,
+                             float* mac32, hipStream_t st);
+hipError_t launch_pops_from_shifted(int64_t n, const ShiftRef& r, const float* pop32, double* pop64, int set,
+                                    hipStream_t st);
+hipError_t launch_pops_to_shifted(int64_t n, const ShiftRef& r, const double* pop64, float* pop32, int set,
+                                  hipStream_t st);
+hipError_t launch_macro_from_shifted(int64_t n, double phi1, const float* mac32, double* mac64, hipStream_t st);
+hipError_t launch_moments_fp32(int64_t n, const float* f, const float* g, float* mac, int which, hipStream_t st);
+hipError_t launch_les_tau_fp32(int nx, int ny, int nz, const Phys& ph, const float* mac, const uint64_t* range,
+                               int slot, float* tau, hipStream_t st);
+
 // populations := equilibria of the macroscopic fields (ini, program/lbm_drop_xy.f90:244-251)
 hipError_t launch_init_equilibrium_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* f,
                                         double* g, hipStream_t st);

@@ -198,6 +198,7 @@ void lattice_free(drp_lattice* L) {
     if (L->force) (void)hipFree(L->force);
     if (L->tg) (void)hipFree(L->tg);
     if (L->partial) (void)hipFree(L->partial);
+    if (L->mac64) (void)hipFree(L->mac64);
     if (L->stream) (void)hipStreamDestroy(L->stream);
     L->stream = nullptr;
 }
@@ -207,7 +208,7 @@ void lattice_free(drp_lattice* L) {
 // (re)start the rotating phi-range slots: slot 0 holds the range used by the
 // first step, (max, min) = (phi2, phi1) as ini_op uses (:278); slot 1 is in the
 // neutral state (max, min) = (phi1, phi2) that physics starts its scan from (:557-558).
-static int reset_range(drp_lattice* L) {
+int reset_range(drp_lattice* L) {
     HIP_TRY(launch_range_set(L->range, 0, L->cfg.phi2, L->cfg.phi1, L->stream));
     HIP_TRY(launch_range_set(L->range, 1, L->cfg.phi1, L->cfg.phi2, L->stream));
     HIP_TRY(launch_range_set(L->range, 2, L->cfg.phi1, L->cfg.phi2, L->stream));
@@ -451,7 +452,7 @@ extern "C" int drp_reduce(drp_lattice* L, drp_stats* st) {
 extern "C" int drp_deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do) {
     if (!L || !Lo || !Bo || !Do) return drp_fail(DRP_ERR_ARG, "null argument");
     if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
-    if (mode != 0 && mode != 1) return drp_fail(DRP_ERR_ARG, "unknown deformation mode %d", mode);
+    if (mode < 0 || mode > 2) return drp_fail(DRP_ERR_ARG, "unknown deformation mode %d", mode);
     return deformation(L, mode, Lo, Bo, Do);
 }
 

@@ -35,6 +35,7 @@ drp::Phys make_phys(const drp_config& c);
 int lattice_alloc(drp_lattice* L);
 void lattice_free(drp_lattice* L);
 drp::StepArgs make_step_args(drp_lattice* L, int64_t nstep);
+int reset_range(drp_lattice* L);
 
 // deformation.cpp
 int deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do);

@@ -1,0 +1,331 @@
+// step_fp32.hip — the D3Q15 two-phase step with FP32 shifted storage.
+//
+// Storage (all FP32):
+//   phi populations      ft_q = f_q - fr_q,   fr_q = H_q phi1 + F_q p0(phi1)
+//   momentum populations gt_q = g_q - E_q     (E_q = g_q at rest with p = 1/3)
+//   fields               phit = phi - phi1, pt = p - 1/3, u1, u2, u3
+// The reference states (the continuous phase at rest) are where the bulk of the
+// domain sits, so the stored deviations are small and FP32 keeps their
+// relative precision.  Sum_q fr_q = phi1 and Sum_q E_q (1, c) = (1, 0), so the
+// moments of the deviations are the deviations of the moments.
+//
+// Arithmetic: equilibria and collision in FP32, written directly in deviation
+// form (no difference of two large numbers):
+//   p0(phi) - p0(phi1) = phit [T / ((1 - b phi)(1 - b phi1)) - a (phi + phi1)]
+//   ft_eq = H phit + F At + 3 E phi (c.u) + E kf gt
+//   gt_eq = E (3 pt + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) + E kg gt
+// with At = [p0(phi) - p0(phi1)] - kf phi lap - kf/6 |grad phi|^2.  The
+// Laplacian and gradient of phi (the chemical-potential terms at the interface)
+// are accumulated in FP64, as is the phi moment; diagnostics reductions run in
+// FP64 on an FP64 copy of the fields.
+//
+// Step structure and reference mapping are those of step_kernels.hip
+// (program/lbm_drop_xy.f90:421-588, prototype_program/two-phase/dropch.f90:666-698).
+#include "d3q15.h"
+#include "kernels.h"
+
+namespace drp {
+
+template <int Q0, int Q1, typename Fn> __device__ __forceinline__ void sfor32(Fn&& fn) {
+    if constexpr (Q0 < Q1) {
+        fn(std::integral_constant<int, Q0>{});
+        sfor32<Q0 + 1, Q1>(fn);
+    }
+}
+
+__device__ __forceinline__ int wrap32(int v, int n, bool mod) {
+    if (mod) {
+        v %= n;
+        return v < 0 ? v + n : v;
+    }
+    return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+
+__device__ __forceinline__ int xcd_tile32(int b, int nb) {
+    if ((nb & 7) != 0) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+__global__ __launch_bounds__(2 * TX* TY* TZ) void step_fp32(StepArgs32 a) {
+    constexpr int NN = TX * TY * TZ, NT = 2 * NN;
+    constexpr int HX2 = TX + 4, HY2 = TY + 4, HZ2 = TZ + 4, H2 = HX2 * HY2 * HZ2;
+    constexpr int HX1 = TX + 2, HY1 = TY + 2, HZ1 = TZ + 2, H1 = HX1 * HY1 * HZ1;
+    static_assert(NN % 64 == 0, "a role must own whole wavefronts");
+
+    __shared__ float s_phi[H2];  // phit
+    __shared__ float s_p[H1], s_u1[H1], s_u2[H1], s_u3[H1];
+    __shared__ float s_A[H1], s_g0[H1], s_g1[H1], s_g2[H1];
+    __shared__ float s_rtau[TAUMODE == 0 ? 1 : H1];  // 1 / tau_g
+    __shared__ double s_red[2][NT / 64];
+
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t N = a.n;
+    const int tile = xcd_tile32(blockIdx.x, gridDim.x);
+    const int x0 = (tile % a.tiles_x) * TX;
+    const int y0 = ((tile / a.tiles_x) % a.tiles_y) * TY;
+    const int z0 = (tile / (a.tiles_x * a.tiles_y)) * TZ;
+    const int tid = threadIdx.x;
+    const bool mom = tid >= NN;
+    const int nid = mom ? tid - NN : tid;
+
+    const float* __restrict__ phi_in = a.mac_in;
+    const float* __restrict__ p_in = a.mac_in + N;
+    const float* __restrict__ u1_in = a.mac_in + 2 * N;
+    const float* __restrict__ u2_in = a.mac_in + 3 * N;
+    const float* __restrict__ u3_in = a.mac_in + 4 * N;
+
+    // phase 0: this thread's 15 upstream populations
+    const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    const bool inside = (x < nx) && (y < ny) && (z < nz);
+    const int xc = inside ? x : 0, yc = inside ? y : 0, zc = inside ? z : 0;
+    const int xs[3] = {xc + 1 == nx ? 0 : xc + 1, xc, xc == 0 ? nx - 1 : xc - 1};
+    const int zs[3] = {zc + 1 == nz ? 0 : zc + 1, zc, zc == 0 ? nz - 1 : zc - 1};
+    const int64_t yo[3] = {(int64_t)nx * (yc + 1 == ny ? 0 : yc + 1), (int64_t)nx * yc,
+                           (int64_t)nx * (yc == 0 ? ny - 1 : yc - 1)};
+    const int64_t nxy = (int64_t)nx * ny;
+    const int64_t zo[3] = {nxy * zs[0], nxy * zs[1], nxy * zs[2]};
+    const float* __restrict__ pin = mom ? a.g_in : a.f_in;
+    float ps[15];
+    if (inside) {
+        sfor32<0, 15>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            using D = Dir<q>;
+            ps[q] = pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]];
+        });
+    }
+
+    // phase 1: stage fields
+    const bool wm = a.wrap_mod != 0;
+    for (int h = tid; h < H2; h += NT) {
+        const int hx = h % HX2, hy = (h / HX2) % HY2, hz = h / (HX2 * HY2);
+        const int gx = wrap32(x0 + hx - 2, nx, wm), gy = wrap32(y0 + hy - 2, ny, wm), gz = wrap32(z0 + hz - 2, nz, wm);
+        s_phi[h] = phi_in[gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
+    }
+    for (int h = tid; h < H1; h += NT) {
+        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
+        const int gx = wrap32(x0 + hx - 1, nx, wm), gy = wrap32(y0 + hy - 1, ny, wm), gz = wrap32(z0 + hz - 1, nz, wm);
+        const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+        s_p[h] = p_in[g];
+        s_u1[h] = u1_in[g];
+        s_u2[h] = u2_in[g];
+        s_u3[h] = u3_in[g];
+        if constexpr (TAUMODE == 2) s_rtau[h] = 1.0f / a.tau_field[g];
+    }
+    __syncthreads();
+
+    // phase 2: FP64-accumulated Laplacian and gradient, deviation F-bracket, 1/tau
+    const double rmax = from_ordered_bits(a.range[2 * a.slot_read + 0]);
+    const double rmin = from_ordered_bits(a.range[2 * a.slot_read + 1]);
+    const float range = (float)(rmax - rmin), rminf = (float)(rmin - a.phi1);
+    for (int h = tid; h < H1; h += NT) {
+        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
+        const int c = (hx + 1) + HX2 * ((hy + 1) + HY2 * (hz + 1));
+        const float pt = s_phi[c];
+        double lap = -14.0 * (double)pt, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        sfor32<1, 15>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            using D = Dir<q>;
+            const double v = (double)s_phi[c + D::cx + HX2 * (D::cy + HY2 * D::cz)];
+            lap += v;
+            g0 = acc<D::cx>(g0, v);
+            g1 = acc<D::cy>(g1, v);
+            g2 = acc<D::cz>(g2, v);
+        });
+        const float lapf = (float)(lap * 0.2), gf0 = (float)(g0 * 0.1), gf1 = (float)(g1 * 0.1),
+                    gf2 = (float)(g2 * 0.1);
+        const float ph = a.phi1f + pt;
+        const float dp0 = pt * (a.Tf / ((1.0f - a.bf * ph) * a.one_m_bphi1) - a.af * (ph + a.phi1f));
+        s_A[h] = dp0 - a.kff * ph * lapf - a.kf6f * (gf0 * gf0 + gf1 * gf1 + gf2 * gf2);
+        s_g0[h] = gf0;
+        s_g1[h] = gf1;
+        s_g2[h] = gf2;
+        if constexpr (TAUMODE == 1) {
+            float frac = (range == 0.0f) ? 0.0f : (pt - rminf) / range;
+            s_rtau[h] = 1.0f / (0.5f + 3.0f * (frac * a.dnuf + a.nu1f));
+        }
+    }
+    __syncthreads();
+
+    // phase 3: collide-stream
+    double vmax = -__builtin_huge_val(), vmin = __builtin_huge_val();
+    if (inside) {
+        const int64_t s = x + (int64_t)nx * (y + (int64_t)ny * z);
+        if (!mom) {
+            double phs = 0.0;
+            sfor32<0, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                constexpr float E = (float)weightE<q>(), Fw = (float)weightF<q>();
+                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                const int h2 = (tx + 2 - D::cx) + HX2 * ((ty + 2 - D::cy) + HY2 * (tz + 2 - D::cz));
+                const float pt = s_phi[h2];
+                float fe = Fw * s_A[hu];
+                if constexpr (q == 0) {
+                    fe += pt;
+                } else {
+                    const float g0 = s_g0[hu], g1 = s_g1[hu], g2 = s_g2[hu];
+                    const float s2h = 1.5f * (g0 * g0 + g1 * g1 + g2 * g2);
+                    const float gt = korteweg<q>(g0, g1, g2, s2h);
+                    const float cu = cdot<q>(s_u1[hu], s_u2[hu], s_u3[hu]);
+                    fe += 3.0f * E * (a.phi1f + pt) * cu + E * a.kff * gt;
+                }
+                const float fv = ps[q] - (ps[q] - fe) * a.rtau_f;
+                phs += (double)fv;
+                __builtin_nontemporal_store(fv, &a.f_out[q * N + s]);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            __builtin_nontemporal_store((float)phs, &a.mac_out[s]);
+            vmax = vmin = phs + a.phi1;
+        } else {
+            sfor32<0, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                constexpr float E = (float)weightE<q>();
+                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                const float u1 = s_u1[hu], u2 = s_u2[hu], u3 = s_u3[hu];
+                const float usq = u1 * u1 + u2 * u2 + u3 * u3;
+                float ge;
+                if constexpr (q == 0) {
+                    ge = E * (3.0f * s_p[hu] - 1.5f * usq);
+                } else {
+                    const float g0 = s_g0[hu], g1 = s_g1[hu], g2 = s_g2[hu];
+                    const float s2h = 1.5f * (g0 * g0 + g1 * g1 + g2 * g2);
+                    const float gt = korteweg<q>(g0, g1, g2, s2h);
+                    const float cu = cdot<q>(u1, u2, u3);
+                    ge = E * (3.0f * s_p[hu] + 3.0f * cu + 4.5f * cu * cu - 1.5f * usq) + E * a.kgf * gt;
+                }
+                const float rt = (TAUMODE == 0) ? a.rtau_const : s_rtau[hu];
+                float gv = ps[q] - (ps[q] - ge) * rt;
+                if constexpr (FORCE != FORCE_NONE && q != 0) {
+                    float Fx, Fy, Fz;
+                    if constexpr (FORCE == FORCE_TAYLOR_GREEN) {
+                        const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
+                        Fx = (float)(a.tg_amp * a.sinx[xu] * a.cosz[zu]);
+                        Fy = 0.0f;
+                        Fz = (float)(-(a.tg_amp * a.cosx[xu] * a.sinz[zu]));
+                    } else {
+                        const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
+                        Fx = (float)a.fx[src];
+                        Fy = (float)a.fy[src];
+                        Fz = (float)a.fz[src];
+                    }
+                    gv += 3.0f * E * ((float)D::cx * Fx + (float)D::cy * Fy + (float)D::cz * Fz);
+                }
+                ps[q] = gv;
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            if constexpr (WALL) {
+                const int coord = (a.wall_axis == 2) ? y : z;
+                const int nax = (a.wall_axis == 2) ? ny : nz;
+                const bool bottom = coord == 0, top = coord == nax - 1;
+                if (bottom || top) {
+                    const float uw = bottom ? -a.wall_u_now : a.wall_u_now;
+                    const int want = bottom ? 1 : -1;
+                    sfor32<1, 15>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        using D = Dir<q>;
+                        constexpr int o = D::opp;
+                        const int ca = (a.wall_axis == 2) ? D::cy : D::cz;
+                        if (ca == want) ps[q] = ps[o] - 6.0f * (float)weightE<o>() * (uw * (float)Dir<o>::cx);
+                    });
+                }
+            }
+            float pr = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+            sfor32<0, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                pr += ps[q];
+                m1 = acc<D::cx>(m1, ps[q]);
+                m2 = acc<D::cy>(m2, ps[q]);
+                m3 = acc<D::cz>(m3, ps[q]);
+                __builtin_nontemporal_store(ps[q], &a.g_out[q * N + s]);
+            });
+            __builtin_nontemporal_store(pr * (1.0f / 3.0f), &a.mac_out[N + s]);
+            __builtin_nontemporal_store(m1, &a.mac_out[2 * N + s]);
+            __builtin_nontemporal_store(m2, &a.mac_out[3 * N + s]);
+            __builtin_nontemporal_store(m3, &a.mac_out[4 * N + s]);
+        }
+    }
+
+    // phi range for nu(phi) of the next step
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+        vmin = fmin(vmin, __shfl_xor(vmin, off, 64));
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) {
+        s_red[0][wave] = vmax;
+        s_red[1][wave] = vmin;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double mx = s_red[0][0], mn = s_red[1][0];
+        for (int w = 1; w < NN / 64; w++) {
+            mx = fmax(mx, s_red[0][w]);
+            mn = fmin(mn, s_red[1][w]);
+        }
+        atomicMax(reinterpret_cast<unsigned long long*>(&a.range[2 * a.slot_write + 0]),
+                  (unsigned long long)ordered_bits(mx));
+        atomicMin(reinterpret_cast<unsigned long long*>(&a.range[2 * a.slot_write + 1]),
+                  (unsigned long long)ordered_bits(mn));
+        if (blockIdx.x == 0) {
+            a.range[2 * a.slot_reset + 0] = ordered_bits(a.phi1);
+            a.range[2 * a.slot_reset + 1] = ordered_bits(a.phi2);
+        }
+    }
+}
+
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUM>
+static hipError_t launch32_shape(const StepArgs32& a, hipStream_t st) {
+    StepArgs32 b = a;
+    b.tiles_x = (a.nx + TX - 1) / TX;
+    b.tiles_y = (a.ny + TY - 1) / TY;
+    b.tiles_z = (a.nz + TZ - 1) / TZ;
+    b.wrap_mod = (a.nx <= TX || a.ny <= TY || a.nz <= TZ) ? 1 : 0;
+    const int nb = b.tiles_x * b.tiles_y * b.tiles_z;
+    hipLaunchKernelGGL((step_fp32<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(nb), dim3(2 * TX * TY * TZ), 0, st, b);
+    return hipGetLastError();
+}
+
+template <int TX, int TY, int TZ, bool WALL, int FORCE>
+static hipError_t launch32_tau(const StepArgs32& a, hipStream_t st) {
+    if (a.tau_field) return launch32_shape<TX, TY, TZ, WALL, FORCE, 2>(a, st);
+    if (a.dnuf == 0.0f) return launch32_shape<TX, TY, TZ, WALL, FORCE, 0>(a, st);
+    return launch32_shape<TX, TY, TZ, WALL, FORCE, 1>(a, st);
+}
+
+template <int TX, int TY, int TZ>
+static hipError_t launch32_variant(const StepArgs32& a, hipStream_t st) {
+    const bool wall = a.wall_axis != 0;
+    switch (a.force_kind) {
+        case FORCE_NONE:
+            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_NONE>(a, st)
+                        : launch32_tau<TX, TY, TZ, false, FORCE_NONE>(a, st);
+        case FORCE_TAYLOR_GREEN:
+            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_TAYLOR_GREEN>(a, st)
+                        : launch32_tau<TX, TY, TZ, false, FORCE_TAYLOR_GREEN>(a, st);
+        case FORCE_FIELD:
+            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_FIELD>(a, st)
+                        : launch32_tau<TX, TY, TZ, false, FORCE_FIELD>(a, st);
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st) {
+    switch (shape) {
+        case SHAPE_16x4x4: return launch32_variant<16, 4, 4>(a, st);
+        case SHAPE_32x4x2: return launch32_variant<32, 4, 2>(a, st);
+        case SHAPE_64x2x2: return launch32_variant<64, 2, 2>(a, st);
+        case SHAPE_32x2x4: return launch32_variant<32, 2, 4>(a, st);
+        case SHAPE_32x4x4: return launch32_variant<32, 4, 4>(a, st);
+        case SHAPE_32x8x2: return launch32_variant<32, 8, 2>(a, st);
+        case SHAPE_64x4x2: return launch32_variant<64, 4, 2>(a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace drp

@@ -160,7 +160,9 @@ int drp_reduce(drp_lattice* lat, drp_stats* stats);
 
 /* Taylor deformation D = (L-B)/(L+B) in the shear plane.
  * mode 0: break_up3 (prototype_program/two-phase/dropch.f90:857-948);
- * mode 1: L = 2 max r, B = 4 area/(pi L) (prototype_program/two-phase/drop.f90:439-463). */
+ * mode 1: L = 2 max r, B = 4 area/(pi L) (prototype_program/two-phase/drop.f90:439-463);
+ * mode 2: axes from the second moments of max(phi-phi1,0) in the plane (no
+ *         reference counterpart; continuous in phi, used to state FP32 error). */
 int drp_deformation(drp_lattice* lat, int mode, double* L, double* B, double* D);
 
 /* ---- measurement and tuning (no reference counterpart: the reference times its

@@ -616,9 +616,41 @@ static int deformation_area(const ora_lattice* L, double* Lout, double* Bout, do
     return 0;
 }
 
+/* second-moment variant (no reference counterpart): weights max(phi-phi1,0)/(phi2-phi1);
+   for a uniform ellipse the principal moments are W a^2/4 and W b^2/4 */
+static int deformation_inertia(const ora_lattice* L, double* Lout, double* Bout, double* Dout) {
+    const drp_config* c = &L->c;
+    int ax = c->wall_axis == DRP_WALL_Y ? DRP_WALL_Y : DRP_WALL_Z;
+    int n1 = c->nx, n2 = (ax == DRP_WALL_Z) ? c->nz : c->ny, n3 = (ax == DRP_WALL_Z) ? c->ny : c->nz;
+    plane_t P = {L, ax, n3 / 2, n1, n2};
+    double W = 0, sx = 0, sz = 0;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+            double w = fmax(plane_phi(&P, i, j) - c->phi1, 0.0) / (c->phi2 - c->phi1);
+            W += w;
+            sx += w * i;
+            sz += w * j;
+        }
+    if (W <= 0) return DRP_ERR_STATE;
+    double xc = sx / W, zc = sz / W, ixx = 0, izz = 0, ixz = 0;
+    for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+            double w = fmax(plane_phi(&P, i, j) - c->phi1, 0.0) / (c->phi2 - c->phi1);
+            ixx += w * (i - xc) * (i - xc);
+            izz += w * (j - zc) * (j - zc);
+            ixz += w * (i - xc) * (j - zc);
+        }
+    double m = 0.5 * (ixx + izz), r = sqrt(0.25 * (ixx - izz) * (ixx - izz) + ixz * ixz);
+    *Lout = 4.0 * sqrt((m + r) / W);
+    *Bout = 4.0 * sqrt(fmax(m - r, 0.0) / W);
+    *Dout = (*Lout - *Bout) / (*Lout + *Bout);
+    return 0;
+}
+
 int ora_deformation(const ora_lattice* L, int mode, double* Lout, double* Bout, double* Dout) {
     if (mode == 0) return deformation_breakup3(L, Lout, Bout, Dout);
     if (mode == 1) return deformation_area(L, Lout, Bout, Dout);
+    if (mode == 2) return deformation_inertia(L, Lout, Bout, Dout);
     return DRP_ERR_ARG;
 }
 

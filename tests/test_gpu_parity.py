@@ -191,7 +191,7 @@ def test_reduce_vs_oracle():
     np.testing.assert_allclose(a["momentum"], b["momentum"], rtol=1e-9, atol=1e-12)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_deformation_vs_oracle(mode):
     N = 48
     cfg = _shear_cfg(N, wall_axis=DRP_WALL_Z, wall_u=0.02)

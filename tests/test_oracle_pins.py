@@ -181,6 +181,8 @@ def test_deformation_of_analytic_ellipse(wall, theta):
     assert abs(D - (a - b) / (a + b)) < 0.03
     L1, B1, D1 = o.deformation(1)
     assert abs(D1 - (a - b) / (a + b)) < 0.06
+    L2, B2, D2 = o.deformation(2)
+    assert abs(L2 - 2 * a) < 0.5 and abs(B2 - 2 * b) < 0.5 and abs(D2 - (a - b) / (a + b)) < 0.01
 
 
 def test_diagnostics_closed_forms():
