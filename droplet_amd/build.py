@@ -61,6 +61,28 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FC = "/opt/rocm/llvm/bin/amdflang"
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compile the iso_c_binding module and the shear driver against the library
+    (droplet_amd/_lib/shear_driver); skipped when amdflang is absent."""
+    if not os.path.exists(FC):
+        return None
+    exe = os.path.join(OUT_DIR, "shear_driver")
+    with tempfile.TemporaryDirectory(prefix="droplet_fortran_") as tmp:
+        cmd = [FC, "-O2", f"-J{tmp}", os.path.join(FORTRAN, "droplet_amd.f90"), os.path.join(FORTRAN, "shear_driver.f90"),
+               f"-L{OUT_DIR}", "-ldroplet_amd", f"-Wl,-rpath,$ORIGIN", "-o", exe + ".tmp"]
+        if verbose:
+            print(" ".join(cmd), file=sys.stderr)
+        r = subprocess.run(cmd, cwd=tmp, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        if r.returncode:
+            raise RuntimeError("Fortran driver build failed:\n" + r.stdout)
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def _wait(item):
     src, p = item
     out, _ = p.communicate()
@@ -76,3 +98,4 @@ def _wait(item):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

@@ -1,0 +1,168 @@
+! droplet_amd.f90 -- iso_c_binding interface to libdroplet_amd.so (include/droplet.h).
+!
+! Gives a Fortran driver written like the reference's `program main`
+! (program/lbm_drop_xy.f90:756-868) the same call surface:
+!   par + allocation      -> drp_create
+!   ini + ini_op          -> drp_init_droplets / drp_init_fields
+!   loop body             -> drp_step
+!   output / outputfg     -> drp_get_macro / drp_get_populations
+!   step_input injection  -> drp_inject_momentum
+!   break_up3, diagnostics-> drp_deformation / drp_reduce
+! Fields are returned in the reference's x-fastest order, i.e. a Fortran array
+! phi(0:nx-1, 0:ny-1, 0:nz-1) receives them directly.
+module droplet_amd
+    use, intrinsic :: iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: DRP_OK = 0
+    integer(c_int), parameter :: DRP_FP64 = 0, DRP_FP32_SHIFTED = 1
+    integer(c_int), parameter :: DRP_WALL_NONE = 0, DRP_WALL_Y = 2, DRP_WALL_Z = 3
+    integer(c_int), parameter :: DRP_FORCE_NONE = 0, DRP_FORCE_TAYLOR_GREEN = 1, DRP_FORCE_FIELD = 2
+    integer(c_int), parameter :: DRP_FIELD_PHI = 0, DRP_FIELD_P = 1, DRP_FIELD_U1 = 2, DRP_FIELD_U2 = 3, &
+                                 DRP_FIELD_U3 = 4
+    integer(c_int), parameter :: DRP_POP_PHI = 0, DRP_POP_MOM = 1
+
+    type, bind(C) :: drp_config
+        integer(c_int32_t) :: nx, ny, nz, precision
+        real(c_double) :: tau_f, nu1, nu2, kappa_f, kappa_g, phi1, phi2, a, b, T, anu
+        integer(c_int32_t) :: wall_axis, wall_start
+        real(c_double) :: wall_u
+        integer(c_int32_t) :: force_kind, force_start
+        real(c_double) :: force_amp, force_k, les_cs, les_delta
+        integer(c_int32_t) :: reserved(8)
+    end type drp_config
+
+    type, bind(C) :: drp_drop
+        real(c_double) :: xc, yc, zc, diameter
+    end type drp_drop
+
+    type, bind(C) :: drp_stats
+        real(c_double) :: phi_mass, phi_min, phi_max, kinetic, dissipation, interface_area, free_energy, p_mean
+        real(c_double) :: momentum(3)
+    end type drp_stats
+
+    interface
+        integer(c_int) function drp_abi_sizes(config, drop, stats) bind(C, name="drp_abi_sizes")
+            import :: c_int, c_size_t
+            integer(c_size_t), intent(out) :: config, drop, stats
+        end function
+        integer(c_int) function drp_config_default(cfg) bind(C, name="drp_config_default")
+            import :: c_int, drp_config
+            type(drp_config), intent(inout) :: cfg
+        end function
+        integer(c_int) function drp_create(cfg, lat) bind(C, name="drp_create")
+            import :: c_int, c_ptr, drp_config
+            type(drp_config), intent(in) :: cfg
+            type(c_ptr), intent(out) :: lat
+        end function
+        integer(c_int) function drp_init_droplets(lat, drops, ndrops, p_init) bind(C, name="drp_init_droplets")
+            import :: c_int, c_ptr, c_double, drp_drop
+            type(c_ptr), value :: lat
+            type(drp_drop), intent(in) :: drops(*)
+            integer(c_int), value :: ndrops
+            real(c_double), value :: p_init
+        end function
+        integer(c_int) function drp_init_fields(lat, phi, p, u1, u2, u3) bind(C, name="drp_init_fields")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lat
+            real(c_double), intent(in) :: phi(*), p(*), u1(*), u2(*), u3(*)
+        end function
+        integer(c_int) function drp_step(lat, nsteps) bind(C, name="drp_step")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: lat
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function drp_sync(lat) bind(C, name="drp_sync")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: lat
+        end function
+        integer(c_int64_t) function drp_step_count(lat) bind(C, name="drp_step_count")
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: lat
+        end function
+        integer(c_int) function drp_get_macro(lat, field, host, n) bind(C, name="drp_get_macro")
+            import :: c_int, c_ptr, c_double, c_size_t
+            type(c_ptr), value :: lat
+            integer(c_int), value :: field
+            real(c_double), intent(out) :: host(*)
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function drp_get_populations(lat, set, host, n) bind(C, name="drp_get_populations")
+            import :: c_int, c_ptr, c_double, c_size_t
+            type(c_ptr), value :: lat
+            integer(c_int), value :: set
+            real(c_double), intent(out) :: host(*)
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function drp_set_populations(lat, set, host, n) bind(C, name="drp_set_populations")
+            import :: c_int, c_ptr, c_double, c_size_t
+            type(c_ptr), value :: lat
+            integer(c_int), value :: set
+            real(c_double), intent(in) :: host(*)
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function drp_inject_momentum(lat, g, n) bind(C, name="drp_inject_momentum")
+            import :: c_int, c_ptr, c_double, c_size_t
+            type(c_ptr), value :: lat
+            real(c_double), intent(in) :: g(*)
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function drp_reduce(lat, stats) bind(C, name="drp_reduce")
+            import :: c_int, c_ptr, drp_stats
+            type(c_ptr), value :: lat
+            type(drp_stats), intent(out) :: stats
+        end function
+        integer(c_int) function drp_deformation(lat, mode, L, B, D) bind(C, name="drp_deformation")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lat
+            integer(c_int), value :: mode
+            real(c_double), intent(out) :: L, B, D
+        end function
+        type(c_ptr) function drp_last_error_c() bind(C, name="drp_last_error")
+            import :: c_ptr
+        end function
+        subroutine drp_destroy(lat) bind(C, name="drp_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: lat
+        end subroutine
+    end interface
+
+contains
+
+    ! Message of the last failure as a Fortran string.
+    function drp_last_error() result(msg)
+        character(len=:), allocatable :: msg
+        character(kind=c_char), pointer :: s(:)
+        integer :: k
+        call c_f_pointer(drp_last_error_c(), s, [512])
+        k = 0
+        do while (k < 512)
+            if (s(k + 1) == c_null_char) exit
+            k = k + 1
+        end do
+        allocate(character(len=k) :: msg)
+        msg = transfer(s(1:k), msg)
+    end function
+
+    ! Stop with the library's message when rc is not DRP_OK.
+    subroutine drp_check(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(*), intent(in) :: where
+        if (rc /= DRP_OK) then
+            write(*, '(a, a, i0, a, a)') where, ' failed (', rc, '): ', drp_last_error()
+            error stop 1
+        end if
+    end subroutine
+
+    ! Check that this module's struct layouts match the compiled library.
+    logical function drp_layout_ok()
+        integer(c_size_t) :: sc, sd, ss
+        type(drp_config) :: c
+        type(drp_drop) :: d
+        type(drp_stats) :: s
+        integer(c_int) :: v
+        v = drp_abi_sizes(sc, sd, ss)
+        drp_layout_ok = (v == 1) .and. (sc == c_sizeof(c)) .and. (sd == c_sizeof(d)) .and. (ss == c_sizeof(s))
+    end function
+
+end module droplet_amd

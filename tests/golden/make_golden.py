@@ -66,9 +66,13 @@ def fnum(v: float) -> str:
 
 
 def run(exe: str, nranks: int, cwd: str) -> None:
+    """Run under mpiexec; stdout (the reference prints its step counter) goes to
+    $DROPLET_GOLDEN_LOG when set, so long runs can be followed."""
     env = dict(os.environ, OMP_NUM_THREADS="1")
-    subprocess.run([MPIEXEC, "-n", str(nranks), exe], cwd=cwd, env=env, check=True,
-                   stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    log = os.environ.get("DROPLET_GOLDEN_LOG")
+    with open(log, "w") if log else open(os.devnull, "w") as out:
+        subprocess.run([MPIEXEC, "-n", str(nranks), exe], cwd=cwd, env=env, check=True,
+                       stdout=out, stderr=subprocess.PIPE)
 
 
 # ------------------------------------------------------------------ periodic program
@@ -204,8 +208,8 @@ CASES = {
     "periodic32": lambda: case_periodic("periodic32", 32, 2000, (2000,), eta=1.0, Re=1.2, Ca=0.05,
                                         Dv=32.0, Dr=0.375, umax=0.01),
     # injected momentum populations + Taylor-Green body force from step 3 on
-    "inject16": lambda: case_injection("inject16", 16, 60, (3, 4, 60), step_input=3, seed=16, eta=1.0,
-                                       Re=0.4, Ca=0.1, Dv=16.0, Dr=0.5, umax=0.02),
+    "inject16": lambda: case_injection("inject16", 16, 60, (3, 4, 60), step_input=3, seed=16, eta=1.5,
+                                       Re=0.8, Ca=2.0, Dv=16.0, Dr=0.5, umax=0.02),
     # moving walls normal to z, 24^3, 30000 steps
     "walls24": lambda: case_walls("walls24", 24, 30000, D=10.0, nu=0.1, uw=0.01, kappag=0.01),
 }
