@@ -58,63 +58,70 @@ int fp32_init_from_host_fields(drp_lattice* L, const double* const* host) {
     return DRP_OK;
 }
 
-int fp32_step(drp_lattice* L, int nsteps) {
+StepArgs32 make_step_args32(drp_lattice* L, int64_t nstep) {
     const drp_config& c = L->cfg;
     const Phys& P = L->phys;
     const int64_t n = L->n;
+    StepArgs32 a{};
+    a.nx = c.nx;
+    a.ny = c.ny;
+    a.nz = c.nz;
+    a.n = n;
+    float* fi = (float*)L->pop[L->cur];
+    float* fo = (float*)L->pop[1 - L->cur];
+    a.f_in = fi;
+    a.g_in = fi + 15 * n;
+    a.f_out = fo;
+    a.g_out = fo + 15 * n;
+    a.mac_in = (const float*)L->mac[L->cur];
+    a.mac_out = (float*)L->mac[1 - L->cur];
+    a.tau_field = P.les ? (const float*)L->tau : nullptr;
+    a.tg_amp = P.tg_amp;
+    a.phi1 = c.phi1;
+    a.phi2 = c.phi2;
+    a.phi1f = (float)c.phi1;
+    a.Tf = (float)c.T;
+    a.af = (float)c.a;
+    a.bf = (float)c.b;
+    a.one_m_bphi1 = (float)(1.0 - c.b * c.phi1);
+    a.kff = (float)c.kappa_f;
+    a.kf6f = (float)(c.kappa_f / 6.0);
+    a.kgf = (float)c.kappa_g;
+    a.rtau_f = (float)(1.0 / c.tau_f);
+    a.rtau_const = (float)(1.0 / (0.5 + 3.0 * c.nu1));
+    a.nu1f = (float)c.nu1;
+    a.dnuf = (float)(c.nu2 - c.nu1);
+    a.wall_axis = c.wall_axis;
+    a.wall_u_now = (float)((nstep >= c.wall_start) ? c.wall_u : 0.0);
+    a.force_kind = FORCE_NONE;
+    if (c.force_kind == DRP_FORCE_TAYLOR_GREEN && nstep >= c.force_start) {
+        a.force_kind = FORCE_TAYLOR_GREEN;
+        a.sinx = L->tg;
+        a.cosx = L->tg + c.nx;
+        a.sinz = L->tg + 2 * c.nx;
+        a.cosz = L->tg + 2 * c.nx + c.nz;
+    } else if (c.force_kind == DRP_FORCE_FIELD && nstep >= c.force_start) {
+        a.force_kind = FORCE_FIELD;
+        a.fx = L->force;
+        a.fy = L->force + n;
+        a.fz = L->force + 2 * n;
+    }
+    a.range = L->range;
+    a.slot_read = (int)((nstep - 1) % 3);
+    a.slot_write = (int)(nstep % 3);
+    a.slot_reset = (int)((nstep + 1) % 3);
+    return a;
+}
+
+int fp32_step(drp_lattice* L, int nsteps) {
+    const drp_config& c = L->cfg;
+    const Phys& P = L->phys;
     for (int k = 0; k < nsteps; k++) {
         const int64_t nstep = L->step + 1;
         if (P.les)
             HIP_TRY32(launch_les_tau_fp32(c.nx, c.ny, c.nz, P, (const float*)L->mac[L->cur], L->range,
                                           (int)((nstep - 1) % 3), (float*)L->tau, L->stream));
-        StepArgs32 a{};
-        a.nx = c.nx;
-        a.ny = c.ny;
-        a.nz = c.nz;
-        a.n = n;
-        float* fi = (float*)L->pop[L->cur];
-        float* fo = (float*)L->pop[1 - L->cur];
-        a.f_in = fi;
-        a.g_in = fi + 15 * n;
-        a.f_out = fo;
-        a.g_out = fo + 15 * n;
-        a.mac_in = (const float*)L->mac[L->cur];
-        a.mac_out = (float*)L->mac[1 - L->cur];
-        a.tau_field = P.les ? (const float*)L->tau : nullptr;
-        a.tg_amp = P.tg_amp;
-        a.phi1 = c.phi1;
-        a.phi2 = c.phi2;
-        a.phi1f = (float)c.phi1;
-        a.Tf = (float)c.T;
-        a.af = (float)c.a;
-        a.bf = (float)c.b;
-        a.one_m_bphi1 = (float)(1.0 - c.b * c.phi1);
-        a.kff = (float)c.kappa_f;
-        a.kf6f = (float)(c.kappa_f / 6.0);
-        a.kgf = (float)c.kappa_g;
-        a.rtau_f = (float)(1.0 / c.tau_f);
-        a.rtau_const = (float)(1.0 / (0.5 + 3.0 * c.nu1));
-        a.nu1f = (float)c.nu1;
-        a.dnuf = (float)(c.nu2 - c.nu1);
-        a.wall_axis = c.wall_axis;
-        a.wall_u_now = (float)((nstep >= c.wall_start) ? c.wall_u : 0.0);
-        a.force_kind = FORCE_NONE;
-        if (c.force_kind == DRP_FORCE_TAYLOR_GREEN && nstep >= c.force_start) {
-            a.force_kind = FORCE_TAYLOR_GREEN;
-            a.sinx = L->tg;
-            a.cosx = L->tg + c.nx;
-            a.sinz = L->tg + 2 * c.nx;
-            a.cosz = L->tg + 2 * c.nx + c.nz;
-        } else if (c.force_kind == DRP_FORCE_FIELD && nstep >= c.force_start) {
-            a.force_kind = FORCE_FIELD;
-            a.fx = L->force;
-            a.fy = L->force + n;
-            a.fz = L->force + 2 * n;
-        }
-        a.range = L->range;
-        a.slot_read = (int)((nstep - 1) % 3);
-        a.slot_write = (int)(nstep % 3);
-        a.slot_reset = (int)((nstep + 1) % 3);
+        StepArgs32 a = make_step_args32(L, nstep);
         HIP_TRY32(launch_step_fp32(a, L->shape, L->stream));
         L->cur = 1 - L->cur;
         L->step = nstep;

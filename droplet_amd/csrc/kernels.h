@@ -7,9 +7,27 @@
 namespace drp {
 
 enum { FORCE_NONE = 0, FORCE_TAYLOR_GREEN = 1, FORCE_FIELD = 2 };
+// batched launch: per-instance step arguments and first-tile offsets in device memory
+struct BatchLaunch {
+    const void* args;      // StepArgs[ninst] or StepArgs32[ninst]
+    const int* tile_start; // [ninst], ascending, tile_start[0] = 0
+    int ninst;
+    int total_tiles;
+};
+
 // tile shapes of the step kernel (nodes per workgroup in x, y, z; two threads per node)
 enum { SHAPE_16x4x4 = 0, SHAPE_32x4x2 = 1, SHAPE_64x2x2 = 2, SHAPE_32x2x4 = 3, SHAPE_32x4x4 = 4, SHAPE_32x8x2 = 5,
        SHAPE_64x4x2 = 6, SHAPE_COUNT = 7 };
+constexpr int SHAPE_DIMS[SHAPE_COUNT][3] = {{16, 4, 4}, {32, 4, 2}, {64, 2, 2}, {32, 2, 4},
+                                            {32, 4, 4}, {32, 8, 2}, {64, 4, 2}};
+
+// tile counts and wrap mode of one lattice for a tile shape
+template <typename A> inline void set_tiles(A& a, int TX, int TY, int TZ) {
+    a.tiles_x = (a.nx + TX - 1) / TX;
+    a.tiles_y = (a.ny + TY - 1) / TY;
+    a.tiles_z = (a.nz + TZ - 1) / TZ;
+    a.wrap_mod = (a.nx <= TX || a.ny <= TY || a.nz <= TZ) ? 1 : 0;
+}
 
 // Physical constants of one lattice as the kernels need them.
 struct Phys {
@@ -47,7 +65,7 @@ struct StepArgs {
     int wrap_mod;         // set by the launcher: a grid extent does not exceed the tile extent
 };
 
-hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st);
+hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st, const BatchLaunch* batch = nullptr);
 
 // FP32 shifted storage (step_fp32.hip): populations and fields as FP32
 // deviations from the continuous phase at rest
@@ -73,7 +91,7 @@ struct StepArgs32 {
     int slot_read, slot_write, slot_reset;
     int wrap_mod;
 };
-hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st);
+hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st, const BatchLaunch* batch = nullptr);
 
 // conversions between FP64 absolute and FP32 shifted representations; fref is
 // the 15 reference phi populations H_q phi1 + F_q p0(phi1) (host-computed)

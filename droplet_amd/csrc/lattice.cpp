@@ -114,7 +114,9 @@ static int pick_shape(const drp_config& c) {
     return SHAPE_32x4x2;
 }
 
-extern "C" int drp_create(const drp_config* cfg, drp_lattice** out) {
+extern "C" int drp_create(const drp_config* cfg, drp_lattice** out) { return lattice_create(cfg, nullptr, out); }
+
+int lattice_create(const drp_config* cfg, hipStream_t stream, drp_lattice** out) {
     if (!out) return drp_fail(DRP_ERR_ARG, "null output handle");
     *out = nullptr;
     int rc = validate(cfg);
@@ -127,6 +129,7 @@ extern "C" int drp_create(const drp_config* cfg, drp_lattice** out) {
     L->phys = make_phys(*cfg);
     L->n = (int64_t)cfg->nx * cfg->ny * cfg->nz;
     L->shape = pick_shape(*cfg);
+    L->stream = stream;
     (void)hipGetDevice(&L->dev);
     rc = lattice_alloc(L);
     if (rc) {
@@ -153,7 +156,10 @@ int lattice_alloc(drp_lattice* L) {
         }
         return DRP_OK;
     };
-    HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+    if (!L->stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&L->stream, hipStreamNonBlocking));
+        L->owns_stream = true;
+    }
     int rc = 0;
     const size_t esz = (L->cfg.precision == DRP_FP64) ? sizeof(double) : sizeof(float);
     for (int k = 0; k < 2 && !rc; k++) {
@@ -199,7 +205,7 @@ void lattice_free(drp_lattice* L) {
     if (L->tg) (void)hipFree(L->tg);
     if (L->partial) (void)hipFree(L->partial);
     if (L->mac64) (void)hipFree(L->mac64);
-    if (L->stream) (void)hipStreamDestroy(L->stream);
+    if (L->stream && L->owns_stream) (void)hipStreamDestroy(L->stream);
     L->stream = nullptr;
 }
 

@@ -13,6 +13,7 @@ struct drp_lattice {
     drp::Phys phys{};
     int dev = 0;
     hipStream_t stream = nullptr;
+    bool owns_stream = false;   // false for members of a batch (shared stream)
     int64_t n = 0;
     int64_t step = 0;      // reference loop counter of the last completed step
     bool initialized = false;
@@ -33,6 +34,7 @@ struct drp_lattice {
 int drp_fail(int code, const char* fmt, ...);
 drp::Phys make_phys(const drp_config& c);
 int lattice_alloc(drp_lattice* L);
+int lattice_create(const drp_config* cfg, hipStream_t stream, drp_lattice** out);
 void lattice_free(drp_lattice* L);
 drp::StepArgs make_step_args(drp_lattice* L, int64_t nstep);
 int reset_range(drp_lattice* L);
@@ -44,6 +46,7 @@ int copy_phi_plane(drp_lattice* L, int third, double* host, int* n1, int* n2);
 // fp32_shifted.cpp
 int fp32_init_from_host_fields(drp_lattice* L, const double* const* host);
 int fp32_step(drp_lattice* L, int nsteps);
+drp::StepArgs32 make_step_args32(drp_lattice* L, int64_t nstep);
 int fp32_get_macro(drp_lattice* L, int field, double* host);
 int fp32_get_populations(drp_lattice* L, int set, double* host);
 int fp32_set_populations(drp_lattice* L, int set, const double* host, int which);

@@ -47,7 +47,7 @@ __device__ __forceinline__ int xcd_tile32(int b, int nb) {
 }
 
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
-__global__ __launch_bounds__(2 * TX* TY* TZ) void step_fp32(StepArgs32 a) {
+__device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     constexpr int NN = TX * TY * TZ, NT = 2 * NN;
     constexpr int HX2 = TX + 4, HY2 = TY + 4, HZ2 = TZ + 4, H2 = HX2 * HY2 * HZ2;
     constexpr int HX1 = TX + 2, HY1 = TY + 2, HZ1 = TZ + 2, H1 = HX1 * HY1 * HZ1;
@@ -61,7 +61,6 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) void step_fp32(StepArgs32 a) {
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t N = a.n;
-    const int tile = xcd_tile32(blockIdx.x, gridDim.x);
     const int x0 = (tile % a.tiles_x) * TX;
     const int y0 = ((tile / a.tiles_x) % a.tiles_y) * TY;
     const int z0 = (tile / (a.tiles_x * a.tiles_y)) * TZ;
@@ -271,59 +270,83 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) void step_fp32(StepArgs32 a) {
                   (unsigned long long)ordered_bits(mx));
         atomicMin(reinterpret_cast<unsigned long long*>(&a.range[2 * a.slot_write + 1]),
                   (unsigned long long)ordered_bits(mn));
-        if (blockIdx.x == 0) {
+        if (tile == 0) {
             a.range[2 * a.slot_reset + 0] = ordered_bits(a.phi1);
             a.range[2 * a.slot_reset + 1] = ordered_bits(a.phi2);
         }
     }
 }
 
+// one lattice: workgroup id -> tile (XCD-contiguous)
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+__global__ __launch_bounds__(2 * TX* TY* TZ) void step_fp32(StepArgs32 a) {
+    step_tile_fp32<TX, TY, TZ, WALL, FORCE, TAUMODE>(a, xcd_tile32(blockIdx.x, gridDim.x));
+}
+
+// many independent lattices in one launch: tile_start[i] is the first global
+// tile of instance i (ascending, tile_start[0] = 0); the instance of a
+// workgroup is found by binary search, its arguments read from global memory
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+__global__ __launch_bounds__(2 * TX* TY* TZ) void step_batch_fp32(const StepArgs32* __restrict__ args, const int* __restrict__ tile_start, int ninst) {
+    const int b = xcd_tile32(blockIdx.x, gridDim.x);
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    step_tile_fp32<TX, TY, TZ, WALL, FORCE, TAUMODE>(args[lo], b - tile_start[lo]);
+}
+
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUM>
-static hipError_t launch32_shape(const StepArgs32& a, hipStream_t st) {
+static hipError_t launch32_shape(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch) {
+    if (batch) {
+        hipLaunchKernelGGL((step_batch_fp32<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(batch->total_tiles),
+                           dim3(2 * TX * TY * TZ), 0, st, (const StepArgs32*)batch->args, batch->tile_start,
+                           batch->ninst);
+        return hipGetLastError();
+    }
     StepArgs32 b = a;
-    b.tiles_x = (a.nx + TX - 1) / TX;
-    b.tiles_y = (a.ny + TY - 1) / TY;
-    b.tiles_z = (a.nz + TZ - 1) / TZ;
-    b.wrap_mod = (a.nx <= TX || a.ny <= TY || a.nz <= TZ) ? 1 : 0;
+    set_tiles(b, TX, TY, TZ);
     const int nb = b.tiles_x * b.tiles_y * b.tiles_z;
     hipLaunchKernelGGL((step_fp32<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(nb), dim3(2 * TX * TY * TZ), 0, st, b);
     return hipGetLastError();
 }
 
 template <int TX, int TY, int TZ, bool WALL, int FORCE>
-static hipError_t launch32_tau(const StepArgs32& a, hipStream_t st) {
-    if (a.tau_field) return launch32_shape<TX, TY, TZ, WALL, FORCE, 2>(a, st);
-    if (a.dnuf == 0.0f) return launch32_shape<TX, TY, TZ, WALL, FORCE, 0>(a, st);
-    return launch32_shape<TX, TY, TZ, WALL, FORCE, 1>(a, st);
+static hipError_t launch32_tau(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch) {
+    if (a.tau_field) return launch32_shape<TX, TY, TZ, WALL, FORCE, 2>(a, st, batch);
+    if (a.dnuf == 0.0f) return launch32_shape<TX, TY, TZ, WALL, FORCE, 0>(a, st, batch);
+    return launch32_shape<TX, TY, TZ, WALL, FORCE, 1>(a, st, batch);
 }
 
 template <int TX, int TY, int TZ>
-static hipError_t launch32_variant(const StepArgs32& a, hipStream_t st) {
+static hipError_t launch32_variant(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch) {
     const bool wall = a.wall_axis != 0;
     switch (a.force_kind) {
         case FORCE_NONE:
-            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_NONE>(a, st)
-                        : launch32_tau<TX, TY, TZ, false, FORCE_NONE>(a, st);
+            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_NONE>(a, st, batch)
+                        : launch32_tau<TX, TY, TZ, false, FORCE_NONE>(a, st, batch);
         case FORCE_TAYLOR_GREEN:
-            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_TAYLOR_GREEN>(a, st)
-                        : launch32_tau<TX, TY, TZ, false, FORCE_TAYLOR_GREEN>(a, st);
+            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_TAYLOR_GREEN>(a, st, batch)
+                        : launch32_tau<TX, TY, TZ, false, FORCE_TAYLOR_GREEN>(a, st, batch);
         case FORCE_FIELD:
-            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_FIELD>(a, st)
-                        : launch32_tau<TX, TY, TZ, false, FORCE_FIELD>(a, st);
+            return wall ? launch32_tau<TX, TY, TZ, true, FORCE_FIELD>(a, st, batch)
+                        : launch32_tau<TX, TY, TZ, false, FORCE_FIELD>(a, st, batch);
         default:
             return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st) {
+hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st, const BatchLaunch* batch) {
     switch (shape) {
-        case SHAPE_16x4x4: return launch32_variant<16, 4, 4>(a, st);
-        case SHAPE_32x4x2: return launch32_variant<32, 4, 2>(a, st);
-        case SHAPE_64x2x2: return launch32_variant<64, 2, 2>(a, st);
-        case SHAPE_32x2x4: return launch32_variant<32, 2, 4>(a, st);
-        case SHAPE_32x4x4: return launch32_variant<32, 4, 4>(a, st);
-        case SHAPE_32x8x2: return launch32_variant<32, 8, 2>(a, st);
-        case SHAPE_64x4x2: return launch32_variant<64, 4, 2>(a, st);
+        case SHAPE_16x4x4: return launch32_variant<16, 4, 4>(a, st, batch);
+        case SHAPE_32x4x2: return launch32_variant<32, 4, 2>(a, st, batch);
+        case SHAPE_64x2x2: return launch32_variant<64, 2, 2>(a, st, batch);
+        case SHAPE_32x2x4: return launch32_variant<32, 2, 4>(a, st, batch);
+        case SHAPE_32x4x4: return launch32_variant<32, 4, 4>(a, st, batch);
+        case SHAPE_32x8x2: return launch32_variant<32, 8, 2>(a, st, batch);
+        case SHAPE_64x4x2: return launch32_variant<64, 4, 2>(a, st, batch);
         default: return hipErrorInvalidValue;
     }
 }

@@ -74,7 +74,7 @@ template <bool WALL, int FORCE> constexpr int waves_per_eu() { return (WALL && F
 // wavefronts per CU, and the shared staging / stencil phases are spread over
 // twice the threads.
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
-__global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<WALL, FORCE>()))) void step_fp64(StepArgs a) {
+__device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     constexpr int NN = TX * TY * TZ, NT = 2 * NN;
     constexpr int HX2 = TX + 4, HY2 = TY + 4, HZ2 = TZ + 4, H2 = HX2 * HY2 * HZ2;
     constexpr int HX1 = TX + 2, HY1 = TY + 2, HZ1 = TZ + 2, H1 = HX1 * HY1 * HZ1;
@@ -88,7 +88,6 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t N = a.n;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int x0 = (tile % a.tiles_x) * TX;
     const int y0 = ((tile / a.tiles_x) % a.tiles_y) * TY;
     const int z0 = (tile / (a.tiles_x * a.tiles_y)) * TZ;
@@ -325,58 +324,82 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(
                   (unsigned long long)ordered_bits(mx));
         atomicMin(reinterpret_cast<unsigned long long*>(&a.range[2 * a.slot_write + 1]),
                   (unsigned long long)ordered_bits(mn));
-        if (blockIdx.x == 0) {
+        if (tile == 0) {
             a.range[2 * a.slot_reset + 0] = ordered_bits(a.phi1);
             a.range[2 * a.slot_reset + 1] = ordered_bits(a.phi2);
         }
     }
 }
 
+// one lattice: workgroup id -> tile (XCD-contiguous)
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+__global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<WALL, FORCE>()))) void step_fp64(StepArgs a) {
+    step_tile_fp64<TX, TY, TZ, WALL, FORCE, TAUMODE>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// many independent lattices in one launch: tile_start[i] is the first global
+// tile of instance i (ascending, tile_start[0] = 0); the instance of a
+// workgroup is found by binary search, its arguments read from global memory
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+__global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<WALL, FORCE>()))) void step_batch_fp64(const StepArgs* __restrict__ args, const int* __restrict__ tile_start, int ninst) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    step_tile_fp64<TX, TY, TZ, WALL, FORCE, TAUMODE>(args[lo], b - tile_start[lo]);
+}
+
 // ------------------------------------------------------------------ launcher
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUM>
-static hipError_t launch_shape(const StepArgs& a, hipStream_t st) {
+static hipError_t launch_shape(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
+    if (batch) {
+        hipLaunchKernelGGL((step_batch_fp64<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(batch->total_tiles),
+                           dim3(2 * TX * TY * TZ), 0, st, (const StepArgs*)batch->args, batch->tile_start,
+                           batch->ninst);
+        return hipGetLastError();
+    }
     StepArgs b = a;
-    b.tiles_x = (a.nx + TX - 1) / TX;
-    b.tiles_y = (a.ny + TY - 1) / TY;
-    b.tiles_z = (a.nz + TZ - 1) / TZ;
-    b.wrap_mod = (a.nx <= TX || a.ny <= TY || a.nz <= TZ) ? 1 : 0;
+    set_tiles(b, TX, TY, TZ);
     const int nb = b.tiles_x * b.tiles_y * b.tiles_z;
     hipLaunchKernelGGL((step_fp64<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(nb), dim3(2 * TX * TY * TZ), 0, st, b);
     return hipGetLastError();
 }
 
 template <int TX, int TY, int TZ, bool WALL, int FORCE>
-static hipError_t launch_tau(const StepArgs& a, hipStream_t st) {
-    if (a.tau_field) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_FIELD>(a, st);
-    if (a.dnu == 0.0) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_CONST>(a, st);
-    return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_PHI>(a, st);
+static hipError_t launch_tau(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
+    if (a.tau_field) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_FIELD>(a, st, batch);
+    if (a.dnu == 0.0) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_CONST>(a, st, batch);
+    return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_PHI>(a, st, batch);
 }
 
 template <int TX, int TY, int TZ>
-static hipError_t launch_variant(const StepArgs& a, hipStream_t st) {
+static hipError_t launch_variant(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
     const bool wall = a.wall_axis != 0;
     switch (a.force_kind) {
         case FORCE_NONE:
-            return wall ? launch_tau<TX, TY, TZ, true, FORCE_NONE>(a, st) : launch_tau<TX, TY, TZ, false, FORCE_NONE>(a, st);
+            return wall ? launch_tau<TX, TY, TZ, true, FORCE_NONE>(a, st, batch) : launch_tau<TX, TY, TZ, false, FORCE_NONE>(a, st, batch);
         case FORCE_TAYLOR_GREEN:
-            return wall ? launch_tau<TX, TY, TZ, true, FORCE_TAYLOR_GREEN>(a, st)
-                        : launch_tau<TX, TY, TZ, false, FORCE_TAYLOR_GREEN>(a, st);
+            return wall ? launch_tau<TX, TY, TZ, true, FORCE_TAYLOR_GREEN>(a, st, batch)
+                        : launch_tau<TX, TY, TZ, false, FORCE_TAYLOR_GREEN>(a, st, batch);
         case FORCE_FIELD:
-            return wall ? launch_tau<TX, TY, TZ, true, FORCE_FIELD>(a, st) : launch_tau<TX, TY, TZ, false, FORCE_FIELD>(a, st);
+            return wall ? launch_tau<TX, TY, TZ, true, FORCE_FIELD>(a, st, batch) : launch_tau<TX, TY, TZ, false, FORCE_FIELD>(a, st, batch);
         default:
             return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st) {
+hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st, const BatchLaunch* batch) {
     switch (shape) {
-        case SHAPE_16x4x4: return launch_variant<16, 4, 4>(a, st);
-        case SHAPE_32x4x2: return launch_variant<32, 4, 2>(a, st);
-        case SHAPE_64x2x2: return launch_variant<64, 2, 2>(a, st);
-        case SHAPE_32x2x4: return launch_variant<32, 2, 4>(a, st);
-        case SHAPE_32x4x4: return launch_variant<32, 4, 4>(a, st);
-        case SHAPE_32x8x2: return launch_variant<32, 8, 2>(a, st);
-        case SHAPE_64x4x2: return launch_variant<64, 4, 2>(a, st);
+        case SHAPE_16x4x4: return launch_variant<16, 4, 4>(a, st, batch);
+        case SHAPE_32x4x2: return launch_variant<32, 4, 2>(a, st, batch);
+        case SHAPE_64x2x2: return launch_variant<64, 2, 2>(a, st, batch);
+        case SHAPE_32x2x4: return launch_variant<32, 2, 4>(a, st, batch);
+        case SHAPE_32x4x4: return launch_variant<32, 4, 4>(a, st, batch);
+        case SHAPE_32x8x2: return launch_variant<32, 8, 2>(a, st, batch);
+        case SHAPE_64x4x2: return launch_variant<64, 4, 2>(a, st, batch);
         default: return hipErrorInvalidValue;
     }
 }

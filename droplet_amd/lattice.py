@@ -62,6 +62,17 @@ def lib():
         L.drp_get_tile_shape.argtypes = [vp]
         L.drp_autotune.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_int), dp]
         L.drp_copy_bandwidth.argtypes = [C.c_size_t, C.c_int, dp]
+        L.drp_batch_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(vp)]
+        L.drp_batch_size.argtypes = [vp]
+        L.drp_batch_member.argtypes = [vp, C.c_int]
+        L.drp_batch_member.restype = vp
+        L.drp_batch_step.argtypes = [vp, C.c_int]
+        L.drp_batch_use_graph.argtypes = [vp, C.c_int]
+        L.drp_batch_set_tile_shape.argtypes = [vp, C.c_int]
+        L.drp_batch_sync.argtypes = [vp]
+        L.drp_batch_time_steps.argtypes = [vp, C.c_int, dp]
+        L.drp_batch_destroy.argtypes = [vp]
+        L.drp_batch_destroy.restype = None
         _lib = L
     return _lib
 
@@ -89,7 +100,7 @@ class Lattice:
 
     def close(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and not getattr(self, "_borrowed", False):
             lib().drp_destroy(h)
         self._h = None
 
@@ -199,3 +210,59 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
     gbs = C.c_double()
     _check(lib().drp_copy_bandwidth(int(nbytes), int(reps), C.byref(gbs)), "drp_copy_bandwidth")
     return gbs.value
+
+
+class Batch:
+    """Independent lattices (any sizes and parameters) advanced by one launch per step."""
+
+    def __init__(self, cfgs):
+        arr = (Config * len(cfgs))(*cfgs)
+        h = C.c_void_p()
+        _check(lib().drp_batch_create(arr, len(cfgs), C.byref(h)), "drp_batch_create")
+        self._h = h
+        self.members = []
+        for i, cfg in enumerate(cfgs):
+            m = Lattice.__new__(Lattice)
+            m.cfg = cfg
+            m.shape = (cfg.nz, cfg.ny, cfg.nx)
+            m.n = cfg.nx * cfg.ny * cfg.nz
+            m._h = C.c_void_p(lib().drp_batch_member(h, i))
+            m._borrowed = True
+            self.members.append(m)
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, i) -> "Lattice":
+        return self.members[i]
+
+    def step(self, n: int = 1):
+        _check(lib().drp_batch_step(self._h, int(n)), "drp_batch_step")
+
+    def sync(self):
+        _check(lib().drp_batch_sync(self._h), "drp_batch_sync")
+
+    def time_steps(self, n: int) -> float:
+        ms = C.c_double()
+        _check(lib().drp_batch_time_steps(self._h, int(n), C.byref(ms)), "drp_batch_time_steps")
+        return ms.value
+
+    def use_graph(self, enable: bool):
+        _check(lib().drp_batch_use_graph(self._h, int(bool(enable))), "drp_batch_use_graph")
+
+    def set_tile_shape(self, shape: int):
+        _check(lib().drp_batch_set_tile_shape(self._h, int(shape)), "drp_batch_set_tile_shape")
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            for m in self.members:
+                m._h = None
+            lib().drp_batch_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -4,6 +4,12 @@
         stream copy of a known size: the calibration kernel for FETCH_SIZE/WRITE_SIZE
     python -m droplet_amd.probe step [--grid 256] [--steps 20] [--fp32] [--shape S]
         the 256^3 shear-droplet step kernel of bench.py, a few launches
+    python -m droplet_amd.probe ensemble [--members 64] [--steps 60] [--fp64]
+        64 shear-droplet members, N uniform in {48, 64, 96, 128}, radius N/5,
+        Ca uniform in [0.05, 0.5], centre offsets within +-2 cells
+        (numpy.random.default_rng(4)), FP32 shifted storage: aggregate MLUPS of
+        one batched launch per step (hipGraph) against stepping the members one
+        after another
 """
 from __future__ import annotations
 
@@ -17,17 +23,69 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 
+def ensemble(a, dl):
+    import numpy as np
+    from droplet_amd.abi import DRP_FP32_SHIFTED, DRP_FP64, DRP_WALL_Y, make_config
+    rng = np.random.default_rng(4)
+    kappa_g = 0.002
+    sigma = 1.7039 * kappa_g ** 0.9991
+    cfgs, drops = [], []
+    for _ in range(a.members):
+        N = int(rng.choice([48, 64, 96, 128]))
+        Ca = float(rng.uniform(0.05, 0.5))
+        D = 2.0 * N / 5.0
+        U = Ca * (N - 1) * sigma / (0.1 * D)   # Ca = U nu D / (H sigma)
+        cfgs.append(make_config(N, precision=DRP_FP64 if a.fp64 else DRP_FP32_SHIFTED, nu1=0.1, nu2=0.1,
+                                kappa_g=kappa_g, wall_axis=DRP_WALL_Y, wall_u=U, wall_start=1))
+        off = rng.uniform(-2.0, 2.0, 3)
+        drops.append([tuple(0.5 * (N - 1) + off) + (D,)])
+    nodes = sum(c.nx * c.ny * c.nz for c in cfgs)
+    b = dl.Batch(cfgs)
+    if a.shape >= 0:
+        b.set_tile_shape(a.shape)
+    for m, d in zip(b.members, drops):
+        m.init_droplets(d)
+    b.step(12)
+    ms_batch = b.time_steps(a.steps)
+    b.close()
+    solos = []
+    for c, d in zip(cfgs, drops):
+        s = dl.Lattice(c)
+        s.init_droplets(d)
+        s.step(2)
+        solos.append(s)
+    for s in solos:
+        s.sync()
+    import time
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        for s in solos:
+            s.step(1)
+    for s in solos:
+        s.sync()
+    ms_loop = (time.perf_counter() - t0) * 1e3
+    print(json.dumps({"members": a.members, "nodes": nodes, "steps": a.steps,
+                      "batch_ms_per_step": ms_batch / a.steps, "batch_MLUPS": nodes * a.steps / ms_batch / 1e3,
+                      "loop_ms_per_step": ms_loop / a.steps, "loop_MLUPS": nodes * a.steps / ms_loop / 1e3,
+                      "speedup": ms_loop / ms_batch}))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "step", "autotune"])
+    ap.add_argument("what", choices=["copy", "step", "autotune", "ensemble"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--shape", type=int, default=-1)
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--fp64", action="store_true")
     a = ap.parse_args(argv)
     from droplet_amd import lattice as dl
+    if a.what == "ensemble":
+        ensemble(a, dl)
+        return
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
         print(json.dumps({"copy_bytes": a.mib << 20, "reps": a.reps, "GB/s": round(gbs, 1)}))

@@ -182,6 +182,29 @@ int drp_get_tile_shape(const drp_lattice* lat);
  * ms_per_step (may be NULL) receives drp_tile_shape_count() timings. */
 int drp_autotune(const drp_config* cfg, int nsteps, int* best_shape, double* ms_per_step);
 
+/* ---- ensembles: many independent lattices, one launch per step ----
+ * Replaces the reference's parameter ensembles run as MPI sub-communicator
+ * groups (prototype_program/two-phase/multidrop2.f90:235-260,
+ * program/LBM.f90:215-232) or as a sequential case loop
+ * (program/lbm_drop_iht_xy_many.f90).  Members may differ in grid size and in
+ * every physical parameter; they must share precision, the wall / body-force /
+ * LES switches, the body-force start step and whether nu1 == nu2.  LES members
+ * are not supported.  Members are initialised and read through their own
+ * drp_lattice handle (drp_batch_member); they are stepped only together. */
+typedef struct drp_batch drp_batch;
+int drp_batch_create(const drp_config* cfgs, int count, drp_batch** out);
+int drp_batch_size(const drp_batch* b);
+/* Borrowed handle of member i (valid until drp_batch_destroy; do not destroy it). */
+drp_lattice* drp_batch_member(drp_batch* b, int i);
+/* Advance every member by nsteps: one launch per step; six-step cycles are
+ * captured once into a hipGraph and replayed (drp_batch_use_graph(b, 0) disables). */
+int drp_batch_step(drp_batch* b, int nsteps);
+int drp_batch_use_graph(drp_batch* b, int enable);
+int drp_batch_set_tile_shape(drp_batch* b, int shape);
+int drp_batch_sync(drp_batch* b);
+int drp_batch_time_steps(drp_batch* b, int nsteps, double* ms);
+void drp_batch_destroy(drp_batch* b);
+
 /* Achievable HBM bandwidth of a plain stream copy of `bytes` (read + write counted),
  * averaged over reps launches on a private stream. */
 int drp_copy_bandwidth(size_t bytes, int reps, double* gbs);
