@@ -1,0 +1,287 @@
+// adjoint.hip — discrete adjoint of the FP64 two-phase step (reverse mode, by hand).
+//
+// Forward step n -> n+1 (step_kernels.hip): at every node y the equilibria
+// feq(phi, p0, lap phi, grad phi, u) and geq(p, u, grad phi) are formed from the
+// stored macroscopic fields m^n = (phi, p, u), the populations relax
+//   f*_q(y) = f_q(y) - (f_q(y) - feq_q(y)) / tau_f,
+//   g*_q(y) = g_q(y) - (g_q(y) - geq_q(y)) / tau_g + force,
+// stream to y + c_q, the momentum populations entering the fluid on a wall
+// layer are overwritten by bounce-back, and m^{n+1} = moments(f^{n+1}, g^{n+1}).
+//
+// With Fh, Gh the adjoints of f^{n+1}, g^{n+1} (total: they include the use
+// of m^{n+1} in the next step), one reverse step is
+//   kernel A (per node y, pulling from y + c_q):
+//     Fh*_q(y) = Fh_q(y + c_q),  Gh*_q(y) = bounce-back transpose of Gh at y + c_q;
+//     adjoints of feq / geq = Fh* / tau_f, Gh* / tau_g  ->  local adjoints of
+//     phi, p, u, the F-bracket A, lap phi and grad phi at y;
+//     Fh^n_q(y) = (1 - 1/tau_f) Fh*_q(y) + phih_local(y)      (stencil part below)
+//     Gh^n_q(y) = (1 - 1/tau_g) Gh*_q(y) + ph(y)/3 + c_q . uh(y)
+//     parameter sensitivities  d/dtau_f, d/dkappa_g, d/dU_wall  (per workgroup)
+//   kernel B (per node x): the transposed lap / grad stencil
+//     phih_st(x) = -14/5 laph(x) + sum_q [laph(x - c_q)/5 + c_q . gradh(x - c_q)/10]
+// and the next reverse step reads Fh^n_q(x) = Fpart_q(x) + phih_st(x).
+//
+// Restrictions (checked on the host): equal viscosities (tau_g constant, so the
+// global phi range never enters), no Smagorinsky model.  Body forces are
+// additive and state independent and so drop out of the adjoint.
+#include "d3q15.h"
+#include "kernels.h"
+
+namespace drp {
+
+namespace {
+constexpr int CX[15] = {0, 1, 0, 0, -1, 0, 0, 1, -1, 1, 1, -1, 1, -1, -1};
+constexpr int CY[15] = {0, 0, 1, 0, 0, -1, 0, 1, 1, -1, 1, -1, -1, 1, -1};
+constexpr int CZ[15] = {0, 0, 0, 1, 0, 0, -1, 1, 1, 1, -1, -1, -1, -1, 1};
+constexpr int OPP[15] = {0, 4, 5, 6, 1, 2, 3, 11, 12, 13, 14, 7, 8, 9, 10};
+constexpr double EW[15] = {2.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                           1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0, 1.0 / 72.0,
+                           1.0 / 72.0};
+constexpr double FW[15] = {-7.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0,
+                           1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0,
+                           1.0 / 24.0};
+constexpr int ADJ_BLOCK = 256;
+
+__device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
+
+struct Node {
+    int x, y, z;
+};
+__device__ __forceinline__ Node coords(int64_t s, int nx, int ny) {
+    Node c;
+    c.x = (int)(s % nx);
+    c.y = (int)((s / nx) % ny);
+    c.z = (int)(s / ((int64_t)nx * ny));
+    return c;
+}
+__device__ __forceinline__ int64_t shifted(const Node& c, int dx, int dy, int dz, int nx, int ny, int nz) {
+    return wrap1(c.x + dx, nx) + (int64_t)nx * (wrap1(c.y + dy, ny) + (int64_t)ny * wrap1(c.z + dz, nz));
+}
+
+// workgroup sum of three partials, added into acc[blockIdx.x][0..2] (each
+// workgroup owns its slot, so the accumulation order is fixed)
+__device__ void block_accumulate(double v0, double v1, double v2, double* acc) {
+    __shared__ double sh[3][ADJ_BLOCK / 64];
+    double v[3] = {v0, v1, v2};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 3; k++) sh[k][wave] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = sh[threadIdx.x][0];
+        for (int w = 1; w < ADJ_BLOCK / 64; w++) t += sh[threadIdx.x][w];
+        acc[blockIdx.x * 3 + threadIdx.x] += t;
+    }
+}
+}  // namespace
+
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_collide_fp64(AdjArgs a) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t n = a.n;
+    const int64_t s = blockIdx.x * (int64_t)ADJ_BLOCK + threadIdx.x;
+    double d_tauf = 0.0, d_kg = 0.0, d_u = 0.0;
+    if (s < n) {
+        const Node c = coords(s, nx, ny);
+        const double phi = a.mac[s];
+        const double u[3] = {a.mac[2 * n + s], a.mac[3 * n + s], a.mac[4 * n + s]};
+        // forward stencils at y (various_cal)
+        double lap = -14.0 * phi, gr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 1; q < 15; q++) {
+            const double v = a.mac[shifted(c, CX[q], CY[q], CZ[q], nx, ny, nz)];
+            lap += v;
+            gr[0] += CX[q] * v;
+            gr[1] += CY[q] * v;
+            gr[2] += CZ[q] * v;
+        }
+        lap /= 5.0;
+        gr[0] /= 10.0;
+        gr[1] /= 10.0;
+        gr[2] /= 10.0;
+        const double gsq = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
+        const double ombp = 1.0 - a.b * phi;
+        const double p0 = phi * a.T / ombp - a.a * phi * phi;
+        const double A = p0 - a.kf * phi * lap - a.kf6 * gsq;
+
+        const int wcoord_self = (a.wall_axis == 2) ? c.y : c.z;
+        const int wn = (a.wall_axis == 2) ? ny : nz;
+
+        double phih = 0.0, Ah = 0.0, ph = 0.0, uh[3] = {0.0, 0.0, 0.0}, gh[3] = {0.0, 0.0, 0.0};
+        double Fp[15], Gp[15], gts[15];
+#pragma unroll
+        for (int q = 0; q < 15; q++) {
+            const int64_t d = shifted(c, CX[q], CY[q], CZ[q], nx, ny, nz);  // downstream node
+            double Fs = a.Fin[q * n + d];
+            if (a.phin_in) Fs += a.phin_in[d];
+            double Gs = a.Gin[q * n + d];
+            if (a.wall_axis != 0) {
+                const int cw = (a.wall_axis == 2) ? wrap1(c.y + CY[q], ny) : wrap1(c.z + CZ[q], nz);
+                const bool bottom = cw == 0, top = cw == wn - 1;
+                if (bottom || top) {
+                    const int want = bottom ? 1 : -1;
+                    const int ca = (a.wall_axis == 2) ? CY[q] : CZ[q];
+                    if (ca == want) Gs = 0.0;                                   // overwritten at d
+                    else if (ca == -want) Gs += a.Gin[OPP[q] * n + d];          // copied into opp(q) at d
+                }
+            }
+            const double cu = CX[q] * u[0] + CY[q] * u[1] + CZ[q] * u[2];
+            const double cg = CX[q] * gr[0] + CY[q] * gr[1] + CZ[q] * gr[2];
+            const double c2 = (double)(CX[q] * CX[q] + CY[q] * CY[q] + CZ[q] * CZ[q]);
+            const double gt = (q == 0) ? 0.0 : 4.5 * cg * cg - 1.5 * gsq * c2;
+            gts[q] = gt;
+            const double fe = ((q == 0) ? phi : 0.0) + FW[q] * A + 3.0 * EW[q] * phi * cu + EW[q] * a.kf * gt;
+            const double fh = Fs / a.tau_f;   // adjoint of feq_q
+            const double geh = Gs / a.tau_g;  // adjoint of geq_q
+            d_tauf += Fs * (a.f[q * n + s] - fe) / (a.tau_f * a.tau_f);
+            d_kg += geh * EW[q] * gt;
+            if (q == 0) phih += fh;
+            Ah += FW[q] * fh;
+            phih += fh * 3.0 * EW[q] * cu;
+            const double cq[3] = {(double)CX[q], (double)CY[q], (double)CZ[q]};
+            const double w = EW[q] * (a.kf * fh + a.kg * geh);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                uh[k] += fh * 3.0 * EW[q] * phi * cq[k] + EW[q] * geh * (3.0 * cq[k] + 9.0 * cu * cq[k] - 3.0 * u[k]);
+                gh[k] += w * (9.0 * cg * cq[k] - 3.0 * c2 * gr[k]);
+            }
+            ph += 3.0 * EW[q] * geh;
+            Fp[q] = (1.0 - 1.0 / a.tau_f) * Fs;
+            Gp[q] = (1.0 - 1.0 / a.tau_g) * Gs;
+        }
+        // F-bracket A = p0(phi) - kf phi lap - kf/6 |grad|^2
+        const double dp0 = a.T / (ombp * ombp) - 2.0 * a.a * phi;
+        phih += Ah * (dp0 - a.kf * lap);
+        const double laph = -a.kf * phi * Ah;
+#pragma unroll
+        for (int k = 0; k < 3; k++) gh[k] += -2.0 * a.kf6 * gr[k] * Ah;
+
+        if (a.first) {
+            // populations of this state are geq / feq of constant initial fields:
+            // only the kappa_g dependence of g^0 = geq(m^0) remains
+#pragma unroll
+            for (int q = 0; q < 15; q++) {
+                d_kg += Gp[q] * EW[q] * gts[q];
+                a.Fout[q * n + s] = Fp[q];
+                a.Gout[q * n + s] = Gp[q];
+            }
+        } else {
+            const double p3 = ph / 3.0;
+#pragma unroll
+            for (int q = 0; q < 15; q++) {
+                a.Fout[q * n + s] = Fp[q] + phih;
+                a.Gout[q * n + s] = Gp[q] + p3 + (CX[q] * uh[0] + CY[q] * uh[1] + CZ[q] * uh[2]);
+            }
+            a.lapA[s] = laph;
+            a.gradA[s] = gh[0];
+            a.gradA[n + s] = gh[1];
+            a.gradA[2 * n + s] = gh[2];
+        }
+
+        // wall speed: on a wall layer g_j = g_opp(j) - 6 E_opp (U_w . c_opp), U_w = (-+U, 0, 0)
+        if (a.wall_axis != 0 && a.wall_active) {
+            const bool bottom = wcoord_self == 0, top = wcoord_self == wn - 1;
+            if (bottom || top) {
+                const int want = bottom ? 1 : -1;
+                const double sgn = bottom ? -1.0 : 1.0;
+#pragma unroll
+                for (int q = 1; q < 15; q++) {
+                    const int ca = (a.wall_axis == 2) ? CY[q] : CZ[q];
+                    if (ca == want) {
+                        const int o = OPP[q];
+                        d_u += a.Gin[q * n + s] * (-6.0 * EW[o] * (double)CX[o] * sgn);
+                    }
+                }
+            }
+        }
+    }
+    block_accumulate(d_tauf, d_kg, d_u, a.acc);
+}
+
+// phih_st(x) = -14/5 laph(x) + sum_{q>=1} [laph(x - c_q)/5 + c_q . gradh(x - c_q)/10]
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_stencil_fp64(int nx, int ny, int nz, const double* __restrict__ lapA,
+                                                              const double* __restrict__ gradA,
+                                                              double* __restrict__ out) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t s = blockIdx.x * (int64_t)ADJ_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const Node c = coords(s, nx, ny);
+    double l = 0.0, g = 0.0;
+#pragma unroll
+    for (int q = 1; q < 15; q++) {
+        const int64_t up = shifted(c, -CX[q], -CY[q], -CZ[q], nx, ny, nz);
+        l += lapA[up];
+        g += CX[q] * gradA[up] + CY[q] * gradA[n + up] + CZ[q] * gradA[2 * n + up];
+    }
+    out[s] = -14.0 / 5.0 * lapA[s] + l / 5.0 + g / 10.0;
+}
+
+// seed of sum_x |grad phi(x)|: gradA = w grad phi / |grad phi| (0 where grad phi = 0), lapA = 0
+__global__ __launch_bounds__(ADJ_BLOCK) void adj_seed_area_fp64(int nx, int ny, int nz, const double* __restrict__ phi,
+                                                                double w, double* __restrict__ lapA,
+                                                                double* __restrict__ gradA) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t s = blockIdx.x * (int64_t)ADJ_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const Node c = coords(s, nx, ny);
+    double gr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 1; q < 15; q++) {
+        const double v = phi[shifted(c, CX[q], CY[q], CZ[q], nx, ny, nz)];
+        gr[0] += CX[q] * v;
+        gr[1] += CY[q] * v;
+        gr[2] += CZ[q] * v;
+    }
+    const double m = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);  // 10 |grad phi|
+    const double sc = m > 0.0 ? w / m : 0.0;
+    lapA[s] = 0.0;
+    gradA[s] = gr[0] * sc;
+    gradA[n + s] = gr[1] * sc;
+    gradA[2 * n + s] = gr[2] * sc;
+}
+
+// out[plane node (i, j)] += vals[j * n1 + i]; the plane is (x, y) at z = third
+// (wall_y) or (x, z) at y = third
+__global__ void adj_add_plane(int nx, int ny, int n1, int n2, int wall_y, int third, const double* __restrict__ vals,
+                              double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n1 * n2) return;
+    const int i = t % n1, j = t / n1;
+    const int64_t s = wall_y ? i + (int64_t)nx * (j + (int64_t)ny * third) : i + (int64_t)nx * (third + (int64_t)ny * j);
+    out[s] += vals[t];
+}
+
+// ------------------------------------------------------------------ launchers
+static inline int nblocks(int64_t n) { return (int)((n + ADJ_BLOCK - 1) / ADJ_BLOCK); }
+
+int adj_block_count(int64_t n) { return nblocks(n); }
+
+hipError_t launch_adj_collide_fp64(const AdjArgs& a, hipStream_t st) {
+    hipLaunchKernelGGL(adj_collide_fp64, dim3(nblocks(a.n)), dim3(ADJ_BLOCK), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_stencil_fp64(int nx, int ny, int nz, const double* lapA, const double* gradA, double* out,
+                                   hipStream_t st) {
+    hipLaunchKernelGGL(adj_stencil_fp64, dim3(nblocks((int64_t)nx * ny * nz)), dim3(ADJ_BLOCK), 0, st, nx, ny, nz,
+                       lapA, gradA, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_seed_area_fp64(int nx, int ny, int nz, const double* phi, double w, double* lapA,
+                                     double* gradA, hipStream_t st) {
+    hipLaunchKernelGGL(adj_seed_area_fp64, dim3(nblocks((int64_t)nx * ny * nz)), dim3(ADJ_BLOCK), 0, st, nx, ny,
+                       nz, phi, w, lapA, gradA);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_add_plane(int nx, int ny, int n1, int n2, int wall_y, int third, const double* vals,
+                                double* out, hipStream_t st) {
+    hipLaunchKernelGGL(adj_add_plane, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, nx, ny, n1, n2, wall_y, third,
+                       vals, out);
+    return hipGetLastError();
+}
+
+}  // namespace drp

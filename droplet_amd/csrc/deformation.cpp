@@ -11,6 +11,7 @@
 // (the reference reads its stale 1-cell halo there).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -167,12 +168,9 @@ static int area_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
     return DRP_OK;
 }
 
-// Second-moment variant: weights w = max(phi - phi1, 0) / (phi2 - phi1) in the
-// search plane, centroid-relative second moments I; for a uniform ellipse with
-// semi-axes a >= b the eigenvalues satisfy l+ / l- = (a / b)^2 and l+ = W a^2 / 4,
-// so L = 4 sqrt(l+ / W), B = 4 sqrt(l- / W).  Continuous in phi, unlike the
-// farthest-point and threshold-crossing steps of break_up3.
-static int inertia_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
+// second moments of the search plane, L, B, D and optionally dD/dphi on the plane
+static int inertia_core(drp_lattice* L, double* Lo, double* Bo, double* Do, std::vector<double>* grad, int* n1,
+                       int* n2, int* third) {
     const drp_config& c = L->cfg;
     const int n3 = (c.wall_axis == DRP_WALL_Y) ? c.nz : c.ny;
     Plane S;
@@ -182,7 +180,7 @@ static int inertia_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
     double W = 0.0, sx = 0.0, sz = 0.0;
     for (int j = 0; j < S.n2; j++)
         for (int i = 0; i < S.n1; i++) {
-            const double w = std::fmax(S.at(i, j) - c.phi1, 0.0) * inv;
+            const double sn = (S.at(i, j) - c.phi1) * inv, w = sn * sn;
             W += w;
             sx += w * i;
             sz += w * j;
@@ -192,7 +190,7 @@ static int inertia_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
     double ixx = 0.0, izz = 0.0, ixz = 0.0;
     for (int j = 0; j < S.n2; j++)
         for (int i = 0; i < S.n1; i++) {
-            const double w = std::fmax(S.at(i, j) - c.phi1, 0.0) * inv;
+            const double sn = (S.at(i, j) - c.phi1) * inv, w = sn * sn;
             const double dx = i - xc, dz = j - zc;
             ixx += w * dx * dx;
             izz += w * dz * dz;
@@ -203,6 +201,46 @@ static int inertia_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
     *Lo = 4.0 * std::sqrt(lp / W);
     *Bo = 4.0 * std::sqrt(lm / W);
     *Do = (*Lo - *Bo) / (*Lo + *Bo);
+    if (!grad) return DRP_OK;
+    // D = (a - b) / (a + b), a = sqrt(lp), b = sqrt(lm) (W cancels).  The centroid
+    // terms drop out of dI/dw because sum w (i - xc) = 0, so
+    // dD/dw(i, j) = Dxx (i - xc)^2 + Dzz (j - zc)^2 + Dxz (i - xc)(j - zc),
+    // and dw/dphi = 2 s / (phi2 - phi1).
+    const double a = std::sqrt(lp), b = std::sqrt(lm), s2 = (a + b) * (a + b);
+    const double dDa = a > 0.0 ? 2.0 * b / s2 / (2.0 * a) : 0.0;   // dD/dlp
+    const double dDb = b > 0.0 ? -2.0 * a / s2 / (2.0 * b) : 0.0;  // dD/dlm
+    const double hd = r > 0.0 ? 0.25 * (ixx - izz) / r : 0.0, hx = r > 0.0 ? ixz / r : 0.0;
+    const double Dxx = dDa * (0.5 + hd) + dDb * (0.5 - hd);
+    const double Dzz = dDa * (0.5 - hd) + dDb * (0.5 + hd);
+    const double Dxz = dDa * hx - dDb * hx;
+    grad->assign((size_t)S.n1 * S.n2, 0.0);
+    for (int j = 0; j < S.n2; j++)
+        for (int i = 0; i < S.n1; i++) {
+            const double sn = (S.at(i, j) - c.phi1) * inv;
+            const double dx = i - xc, dz = j - zc;
+            (*grad)[(size_t)j * S.n1 + i] = (Dxx * dx * dx + Dzz * dz * dz + Dxz * dx * dz) * 2.0 * sn * inv;
+        }
+    *n1 = S.n1;
+    *n2 = S.n2;
+    *third = n3 / 2;
+    return DRP_OK;
+}
+
+// second-moment variant: weights w = s^2, s = (phi - phi1) / (phi2 - phi1), on the
+// search plane (0 / 1 outside / inside a sharp drop, smooth in phi), centroid-relative second moments I; for a uniform ellipse with
+// semi-axes a >= b the eigenvalues satisfy l+ / l- = (a / b)^2 and l+ = W a^2 / 4,
+// so L = 4 sqrt(l+ / W), B = 4 sqrt(l- / W).  Continuous in phi, unlike the
+// farthest-point and threshold-crossing steps of break_up3.
+static int inertia_variant(drp_lattice* L, double* Lo, double* Bo, double* Do) {
+    return inertia_core(L, Lo, Bo, Do, nullptr, nullptr, nullptr, nullptr);
+}
+
+int deformation_inertia_grad(drp_lattice* L, double* D, double* grad, int* n1, int* n2, int* third) {
+    double Lo, Bo;
+    std::vector<double> g;
+    int rc = inertia_core(L, &Lo, &Bo, D, &g, n1, n2, third);
+    if (rc) return rc;
+    std::copy(g.begin(), g.end(), grad);
     return DRP_OK;
 }
 

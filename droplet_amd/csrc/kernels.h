@@ -126,4 +126,32 @@ hipError_t launch_reduce_fp64(int nx, int ny, int nz, const Phys& ph, const doub
 // fill a range slot with (max, min) = (vmax, vmin)
 hipError_t launch_range_set(uint64_t* range, int slot, double vmax, double vmin, hipStream_t st);
 
+// adjoint.hip: one reverse step of the FP64 step (equal viscosities, no LES)
+struct AdjArgs {
+    int nx, ny, nz;
+    int64_t n;
+    const double* f;        // forward phi populations at step n (15n)
+    const double* mac;      // forward macroscopic fields at step n (5n)
+    const double* Fin;      // adjoint of f^{n+1} without its stencil part (15n)
+    const double* phin_in;  // stencil part of the adjoint of phi^{n+1} (n), or nullptr
+    const double* Gin;      // adjoint of g^{n+1} (15n)
+    double* Fout;           // adjoint of f^n without its stencil part
+    double* Gout;           // adjoint of g^n
+    double* lapA;           // adjoint of lap phi at step n (n)
+    double* gradA;          // adjoint of grad phi at step n (3n)
+    double* acc;            // [workgroups][3]: d/dtau_f, d/dkappa_g, d/dU_wall, accumulated
+    double tau_f, tau_g, kf, kf6, kg, T, a, b;
+    int wall_axis;          // 0, 2 (y) or 3 (z)
+    int wall_active;        // the walls move during step n -> n+1
+    int first;              // step n is the initial equilibrium of constant fields
+};
+int adj_block_count(int64_t n);
+hipError_t launch_adj_collide_fp64(const AdjArgs& a, hipStream_t st);
+hipError_t launch_adj_stencil_fp64(int nx, int ny, int nz, const double* lapA, const double* gradA, double* out,
+                                   hipStream_t st);
+hipError_t launch_adj_seed_area_fp64(int nx, int ny, int nz, const double* phi, double w, double* lapA,
+                                     double* gradA, hipStream_t st);
+hipError_t launch_adj_add_plane(int nx, int ny, int n1, int n2, int wall_y, int third, const double* vals,
+                                double* out, hipStream_t st);
+
 }  // namespace drp

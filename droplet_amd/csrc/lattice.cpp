@@ -235,6 +235,7 @@ static int init_from_host_fields(drp_lattice* L, const double* const* host) {
     HIP_TRY(hipStreamSynchronize(L->stream));
     L->step = 0;
     L->initialized = true;
+    L->eq_init = true;
     return DRP_OK;
 }
 
@@ -384,6 +385,7 @@ static int set_pops(drp_lattice* L, int set, const double* host, size_t n, int w
     if (L->cfg.precision != DRP_FP64) return fp32_set_populations(L, set, host, which);
     double* base = (double*)L->pop[L->cur];
     double* dst = base + (set == DRP_POP_MOM ? 15 * L->n : 0);
+    L->eq_init = false;
     HIP_TRY(hipMemcpyAsync(dst, host, n * sizeof(double), hipMemcpyHostToDevice, L->stream));
     HIP_TRY(launch_moments_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, base, base + 15 * L->n, (double*)L->mac[L->cur],
                                 which, L->stream));
@@ -436,9 +438,15 @@ extern "C" int drp_reduce(drp_lattice* L, drp_stats* st) {
     HIP_TRY(hipMemcpyAsync(part.data(), L->partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIP_TRY(hipMemcpyAsync(rg, L->range + 2 * (L->step % 3), 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
-    double s[NPART] = {0};
+    // Neumaier-compensated sum of the workgroup partials (thousands of terms)
+    double s[NPART] = {0}, comp[NPART] = {0};
     for (int b = 0; b < nblocks; b++)
-        for (int k = 0; k < NPART; k++) s[k] += part[(size_t)b * NPART + k];
+        for (int k = 0; k < NPART; k++) {
+            const double v = part[(size_t)b * NPART + k], t = s[k] + v;
+            comp[k] += (std::fabs(s[k]) >= std::fabs(v)) ? (s[k] - t) + v : (v - t) + s[k];
+            s[k] = t;
+        }
+    for (int k = 0; k < NPART; k++) s[k] += comp[k];
     const double nn = (double)L->n;
     std::memset(st, 0, sizeof(*st));
     st->phi_mass = s[0];

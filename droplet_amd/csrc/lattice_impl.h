@@ -17,6 +17,7 @@ struct drp_lattice {
     int64_t n = 0;
     int64_t step = 0;      // reference loop counter of the last completed step
     bool initialized = false;
+    bool eq_init = false;  // populations are the equilibria of fields set by drp_init_* (no step since)
     int shape = 0;         // tile shape of the step kernel
     int cur = 0;           // which ping-pong buffer holds the current state
     void* pop[2] = {nullptr, nullptr};  // FP64: [f(15n) | g(15n)] doubles; FP32: floats
@@ -41,6 +42,9 @@ int reset_range(drp_lattice* L);
 
 // deformation.cpp
 int deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do);
+// second-moment deformation (mode 2) of the current state and dD/dphi on its plane
+// (n1 x n2 values, plane index third of the axis that is neither x nor wall-normal)
+int deformation_inertia_grad(drp_lattice* L, double* D, double* grad, int* n1, int* n2, int* third);
 int copy_phi_plane(drp_lattice* L, int third, double* host, int* n1, int* n2);
 
 // fp32_shifted.cpp

@@ -73,6 +73,14 @@ def lib():
         L.drp_batch_time_steps.argtypes = [vp, C.c_int, dp]
         L.drp_batch_destroy.argtypes = [vp]
         L.drp_batch_destroy.restype = None
+        L.drp_tape_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.drp_tape_step.argtypes = [vp, C.c_int]
+        L.drp_tape_loss.argtypes = [vp, dp, dp, dp, dp]
+        L.drp_tape_backward.argtypes = [vp, dp, dp, dp]
+        L.drp_tape_bytes.argtypes = [vp]
+        L.drp_tape_bytes.restype = C.c_int64
+        L.drp_tape_destroy.argtypes = [vp]
+        L.drp_tape_destroy.restype = None
         _lib = L
     return _lib
 
@@ -259,6 +267,51 @@ class Batch:
             for m in self.members:
                 m._h = None
             lib().drp_batch_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tape:
+    """Records the FP64 run of a lattice with a checkpoint every K steps; backward()
+    returns dJ/d(mobility, kappa_g, wall speed) of J = w0 D + w1 A (drp_tape_*)."""
+
+    def __init__(self, lat: Lattice, checkpoint_every: int):
+        h = C.c_void_p()
+        _check(lib().drp_tape_create(lat._h, int(checkpoint_every), C.byref(h)), "drp_tape_create")
+        self._h = h
+        self.lattice = lat   # keeps the lattice alive while the tape exists
+
+    def step(self, n: int = 1):
+        _check(lib().drp_tape_step(self._h, int(n)), "drp_tape_step")
+
+    def loss(self, w=(1.0, 1.0)):
+        """(J, D, A) of the current state."""
+        wv = (C.c_double * 2)(*w)
+        J, D, A = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().drp_tape_loss(self._h, wv, C.byref(J), C.byref(D), C.byref(A)), "drp_tape_loss")
+        return J.value, D.value, A.value
+
+    def backward(self, w=(1.0, 1.0)):
+        """(J, [dJ/dM, dJ/dkappa_g, dJ/dU]) by the checkpointed adjoint sweep."""
+        wv = (C.c_double * 2)(*w)
+        J = C.c_double()
+        g = (C.c_double * 3)()
+        _check(lib().drp_tape_backward(self._h, wv, C.byref(J), g), "drp_tape_backward")
+        return J.value, list(g)
+
+    @property
+    def nbytes(self) -> int:
+        return int(lib().drp_tape_bytes(self._h))
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().drp_tape_destroy(h)
         self._h = None
 
     def __del__(self):

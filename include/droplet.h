@@ -161,8 +161,9 @@ int drp_reduce(drp_lattice* lat, drp_stats* stats);
 /* Taylor deformation D = (L-B)/(L+B) in the shear plane.
  * mode 0: break_up3 (prototype_program/two-phase/dropch.f90:857-948);
  * mode 1: L = 2 max r, B = 4 area/(pi L) (prototype_program/two-phase/drop.f90:439-463);
- * mode 2: axes from the second moments of max(phi-phi1,0) in the plane (no
- *         reference counterpart; continuous in phi, used to state FP32 error). */
+ * mode 2: axes from the second moments of s^2, s = (phi-phi1)/(phi2-phi1), in the
+ *         plane (no reference counterpart; smooth in phi, used to state FP32 error
+ *         and as the differentiable deformation of drp_tape_*). */
 int drp_deformation(drp_lattice* lat, int mode, double* L, double* B, double* D);
 
 /* ---- measurement and tuning (no reference counterpart: the reference times its
@@ -204,6 +205,33 @@ int drp_batch_set_tile_shape(drp_batch* b, int shape);
 int drp_batch_sync(drp_batch* b);
 int drp_batch_time_steps(drp_batch* b, int nsteps, double* ms);
 void drp_batch_destroy(drp_batch* b);
+
+/* ---- reverse-mode sensitivities of a run (hand-written discrete adjoint) ----
+ * No counterpart in the reference, which has no differentiation (parameters are
+ * calibrated there by re-running cases, e.g. the kappag sweep written to
+ * stable5.d in prototype_program/two-phase/dropch.f90:985-1000).  A tape records
+ * the FP64 run of one lattice from its current state, keeping a checkpoint of
+ * the full state every K steps; drp_tape_backward recomputes each K-step segment
+ * from its checkpoint and runs the exact adjoint of every step backwards.
+ *
+ * Loss J = w[0] * D + w[1] * A with D the second-moment Taylor deformation
+ * (drp_deformation mode 2) and A = sum |grad phi| (drp_stats.interface_area).
+ * Gradient entries (DRP_GRAD_*): the phi-population mobility M = tau_f - 1/2
+ * (Inamuro model, lattice units: dJ/dM = dJ/dtau_f), kappa_g (surface tension,
+ * sigma = 1.7039 kappa_g^0.9991) and the wall speed U (shear rate 2U/H).
+ * Requirements: FP64, nu1 == nu2, no LES.  When the tape starts right after
+ * drp_init_droplets / drp_init_fields, the kappa_g dependence of the initial
+ * momentum equilibria is included.  The lattice must be stepped only through
+ * the tape while it is recording. */
+enum { DRP_GRAD_MOBILITY = 0, DRP_GRAD_KAPPA_G = 1, DRP_GRAD_WALL_U = 2, DRP_GRAD_COUNT = 3 };
+typedef struct drp_tape drp_tape;
+int drp_tape_create(drp_lattice* lat, int checkpoint_every, drp_tape** out);
+int drp_tape_step(drp_tape* t, int nsteps);
+int drp_tape_loss(drp_tape* t, const double w[2], double* loss, double* D, double* area);
+int drp_tape_backward(drp_tape* t, const double w[2], double* loss, double grad[DRP_GRAD_COUNT]);
+/* Device bytes held by the tape's checkpoints and work buffers. */
+int64_t drp_tape_bytes(const drp_tape* t);
+void drp_tape_destroy(drp_tape* t);
 
 /* Achievable HBM bandwidth of a plain stream copy of `bytes` (read + write counted),
  * averaged over reps launches on a private stream. */

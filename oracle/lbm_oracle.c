@@ -616,8 +616,9 @@ static int deformation_area(const ora_lattice* L, double* Lout, double* Bout, do
     return 0;
 }
 
-/* second-moment variant (no reference counterpart): weights max(phi-phi1,0)/(phi2-phi1);
-   for a uniform ellipse the principal moments are W a^2/4 and W b^2/4 */
+/* second-moment variant (no reference counterpart): weights s^2, s = (phi-phi1)/(phi2-phi1)
+   (smooth in phi; 0 / 1 outside / inside a sharp drop); for a uniform ellipse the
+   principal moments are W a^2/4 and W b^2/4 */
 static int deformation_inertia(const ora_lattice* L, double* Lout, double* Bout, double* Dout) {
     const drp_config* c = &L->c;
     int ax = c->wall_axis == DRP_WALL_Y ? DRP_WALL_Y : DRP_WALL_Z;
@@ -626,7 +627,7 @@ static int deformation_inertia(const ora_lattice* L, double* Lout, double* Bout,
     double W = 0, sx = 0, sz = 0;
     for (int j = 0; j < n2; j++)
         for (int i = 0; i < n1; i++) {
-            double w = fmax(plane_phi(&P, i, j) - c->phi1, 0.0) / (c->phi2 - c->phi1);
+            double sn = (plane_phi(&P, i, j) - c->phi1) / (c->phi2 - c->phi1), w = sn * sn;
             W += w;
             sx += w * i;
             sz += w * j;
@@ -635,7 +636,7 @@ static int deformation_inertia(const ora_lattice* L, double* Lout, double* Bout,
     double xc = sx / W, zc = sz / W, ixx = 0, izz = 0, ixz = 0;
     for (int j = 0; j < n2; j++)
         for (int i = 0; i < n1; i++) {
-            double w = fmax(plane_phi(&P, i, j) - c->phi1, 0.0) / (c->phi2 - c->phi1);
+            double sn = (plane_phi(&P, i, j) - c->phi1) / (c->phi2 - c->phi1), w = sn * sn;
             ixx += w * (i - xc) * (i - xc);
             izz += w * (j - zc) * (j - zc);
             ixz += w * (i - xc) * (j - zc);
