@@ -194,6 +194,8 @@ extern "C" int drp_batch_create(const drp_config* cfgs, int count, drp_batch** o
     }
     if (cfgs[0].les_cs != 0.0 && cfgs[0].les_delta != 0.0)
         return drp_fail(DRP_ERR_UNSUPPORTED, "LES members are not supported in a batch");
+    if (cfgs[0].force_kind == DRP_FORCE_SPECTRAL)
+        return drp_fail(DRP_ERR_UNSUPPORTED, "spectral forcing is not supported in a batch");
     drp_batch* B = new (std::nothrow) drp_batch();
     if (!B) return drp_fail(DRP_ERR_ALLOC, "host allocation failed");
     B->precision = cfgs[0].precision;

@@ -100,7 +100,7 @@ StepArgs32 make_step_args32(drp_lattice* L, int64_t nstep) {
         a.cosx = L->tg + c.nx;
         a.sinz = L->tg + 2 * c.nx;
         a.cosz = L->tg + 2 * c.nx + c.nz;
-    } else if (c.force_kind == DRP_FORCE_FIELD && nstep >= c.force_start) {
+    } else if ((c.force_kind == DRP_FORCE_FIELD || c.force_kind == DRP_FORCE_SPECTRAL) && nstep >= c.force_start) {
         a.force_kind = FORCE_FIELD;
         a.fx = L->force;
         a.fy = L->force + n;
@@ -121,6 +121,8 @@ int fp32_step(drp_lattice* L, int nsteps) {
         if (P.les)
             HIP_TRY32(launch_les_tau_fp32(c.nx, c.ny, c.nz, P, (const float*)L->mac[L->cur], L->range,
                                           (int)((nstep - 1) % 3), (float*)L->tau, L->stream));
+        int rc = spectral_update(L, nstep);
+        if (rc) return rc;
         StepArgs32 a = make_step_args32(L, nstep);
         HIP_TRY32(launch_step_fp32(a, L->shape, L->stream));
         L->cur = 1 - L->cur;

@@ -126,6 +126,21 @@ hipError_t launch_reduce_fp64(int nx, int ny, int nz, const Phys& ph, const doub
 // fill a range slot with (max, min) = (vmax, vmin)
 hipError_t launch_range_set(uint64_t* range, int slot, double vmax, double vmin, hipStream_t st);
 
+// spectral.hip: linear forcing of the shells 1 <= |k| < kc from the velocity field
+struct SpectralWork {
+    double2 *A, *B, *C, *G;   // transform stages (A also holds the inverse row stage)
+    double2 *twx, *twy, *twz; // e^{2 pi i j / n} tables
+    double* elow;             // energy of the forced shells (last call)
+};
+size_t spectral_work_doubles(int nx, int ny, int nz, int kc);
+void spectral_work_layout(SpectralWork& w, double* base, int nx, int ny, int nz, int kc);
+hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const double* u1,
+                                 const double* u2, const double* u3, double* fx, double* fy, double* fz,
+                                 hipStream_t st);
+hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const float* u1,
+                                 const float* u2, const float* u3, double* fx, double* fy, double* fz,
+                                 hipStream_t st);
+
 // adjoint.hip: one reverse step of the FP64 step (equal viscosities, no LES)
 struct AdjArgs {
     int nx, ny, nz;

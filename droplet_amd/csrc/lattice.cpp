@@ -96,8 +96,16 @@ static int validate(const drp_config* c) {
     if (!(c->nu1 > 0.0) || !(c->nu2 > 0.0)) return drp_fail(DRP_ERR_ARG, "viscosities must be positive");
     if (c->wall_axis != DRP_WALL_NONE && c->wall_axis != DRP_WALL_Y && c->wall_axis != DRP_WALL_Z)
         return drp_fail(DRP_ERR_ARG, "wall_axis must be 0, 2 or 3");
-    if (c->force_kind < DRP_FORCE_NONE || c->force_kind > DRP_FORCE_FIELD)
+    if (c->force_kind < DRP_FORCE_NONE || c->force_kind > DRP_FORCE_SPECTRAL)
         return drp_fail(DRP_ERR_ARG, "unknown force_kind %d", c->force_kind);
+    if (c->force_kind == DRP_FORCE_SPECTRAL) {
+        const double kc = c->force_k;
+        if (!(kc == 2.0 || kc == 3.0 || kc == 4.0 || kc == 5.0))
+            return drp_fail(DRP_ERR_UNSUPPORTED, "spectral forcing cutoff force_k must be 2, 3, 4 or 5");
+        if (!(c->force_amp >= 0.0)) return drp_fail(DRP_ERR_ARG, "spectral forcing needs force_amp = epsilon >= 0");
+        if (2 * ((int)kc - 1) >= c->nx || 2 * ((int)kc - 1) >= c->ny || 2 * ((int)kc - 1) >= c->nz)
+            return drp_fail(DRP_ERR_ARG, "grid too small for the forced shells");
+    }
     if (c->anu != 0.0)
         return drp_fail(DRP_ERR_UNSUPPORTED, "anu != 0 (velocity-gradient term of geq) is not implemented");
     for (int k = 0; k < 8; k++)
@@ -168,9 +176,28 @@ int lattice_alloc(drp_lattice* L) {
     }
     if (!rc) rc = dalloc((void**)&L->range, 6 * sizeof(uint64_t));
     if (!rc && L->phys.les) rc = dalloc((void**)&L->tau, (size_t)n * sizeof(double));
-    if (!rc && L->cfg.force_kind == DRP_FORCE_FIELD) {
+    if (!rc && (L->cfg.force_kind == DRP_FORCE_FIELD || L->cfg.force_kind == DRP_FORCE_SPECTRAL)) {
         rc = dalloc((void**)&L->force, 3 * (size_t)n * sizeof(double));
         if (!rc) HIP_TRY(hipMemsetAsync(L->force, 0, 3 * (size_t)n * sizeof(double), L->stream));
+    }
+    if (!rc && L->cfg.force_kind == DRP_FORCE_SPECTRAL) {
+        const int nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz, kc = (int)L->cfg.force_k;
+        rc = dalloc((void**)&L->spec, spectral_work_doubles(nx, ny, nz, kc) * sizeof(double));
+        if (!rc) {
+            spectral_work_layout(L->specw, L->spec, nx, ny, nz, kc);
+            // e^{2 pi i j / n}, j = 0..n-1, per axis, evaluated on the host
+            const int ns[3] = {nx, ny, nz};
+            double2* dst[3] = {L->specw.twx, L->specw.twy, L->specw.twz};
+            const double pi = std::acos(-1.0);
+            for (int a = 0; a < 3; a++) {
+                std::vector<double2> t(ns[a]);
+                for (int j = 0; j < ns[a]; j++) {
+                    const double th = 2.0 * pi * (double)j / (double)ns[a];
+                    t[j] = make_double2(std::cos(th), std::sin(th));
+                }
+                HIP_TRY(hipMemcpy(dst[a], t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+            }
+        }
     }
     if (!rc && L->cfg.force_kind == DRP_FORCE_TAYLOR_GREEN) {
         const int nx = L->cfg.nx, nz = L->cfg.nz;
@@ -202,6 +229,7 @@ void lattice_free(drp_lattice* L) {
     if (L->range) (void)hipFree(L->range);
     if (L->tau) (void)hipFree(L->tau);
     if (L->force) (void)hipFree(L->force);
+    if (L->spec) (void)hipFree(L->spec);
     if (L->tg) (void)hipFree(L->tg);
     if (L->partial) (void)hipFree(L->partial);
     if (L->mac64) (void)hipFree(L->mac64);
@@ -312,7 +340,7 @@ StepArgs make_step_args(drp_lattice* L, int64_t nstep) {
         a.cosx = L->tg + c.nx;
         a.sinz = L->tg + 2 * c.nx;
         a.cosz = L->tg + 2 * c.nx + c.nz;
-    } else if (c.force_kind == DRP_FORCE_FIELD && nstep >= c.force_start) {
+    } else if ((c.force_kind == DRP_FORCE_FIELD || c.force_kind == DRP_FORCE_SPECTRAL) && nstep >= c.force_start) {
         a.force_kind = FORCE_FIELD;
         a.fx = L->force;
         a.fy = L->force + n;
@@ -335,10 +363,31 @@ extern "C" int drp_step(drp_lattice* L, int nsteps) {
         if (L->phys.les)
             HIP_TRY(launch_les_tau_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, L->phys, (const double*)L->mac[L->cur],
                                         L->range, (int)((nstep - 1) % 3), L->tau, L->stream));
+        int rc = spectral_update(L, nstep);
+        if (rc) return rc;
         StepArgs a = make_step_args(L, nstep);
         HIP_TRY(launch_step_fp64(a, L->shape, L->stream));
         L->cur = 1 - L->cur;
         L->step = nstep;
+    }
+    return DRP_OK;
+}
+
+// externalforce at the start of loop iteration n >= step_input, from the
+// velocity of the current state (program/lbm_drop_iht_xy.f90:1008-1014)
+int spectral_update(drp_lattice* L, int64_t nstep) {
+    const drp_config& c = L->cfg;
+    if (c.force_kind != DRP_FORCE_SPECTRAL || nstep < c.force_start) return DRP_OK;
+    const int64_t n = L->n;
+    double *fx = L->force, *fy = L->force + n, *fz = L->force + 2 * n;
+    if (c.precision == DRP_FP64) {
+        const double* m = (const double*)L->mac[L->cur];
+        HIP_TRY(launch_spectral_force(L->specw, c.nx, c.ny, c.nz, (int)c.force_k, c.force_amp, m + 2 * n, m + 3 * n,
+                                      m + 4 * n, fx, fy, fz, L->stream));
+    } else {
+        const float* m = (const float*)L->mac[L->cur];
+        HIP_TRY(launch_spectral_force(L->specw, c.nx, c.ny, c.nz, (int)c.force_k, c.force_amp, m + 2 * n, m + 3 * n,
+                                      m + 4 * n, fx, fy, fz, L->stream));
     }
     return DRP_OK;
 }
@@ -408,6 +457,17 @@ extern "C" int drp_set_force(drp_lattice* L, const double* fx, const double* fy,
     const double* src[3] = {fx, fy, fz};
     for (int k = 0; k < 3; k++)
         HIP_TRY(hipMemcpyAsync(L->force + k * L->n, src[k], n * sizeof(double), hipMemcpyHostToDevice, L->stream));
+    HIP_TRY(hipStreamSynchronize(L->stream));
+    return DRP_OK;
+}
+
+extern "C" int drp_get_force(drp_lattice* L, double* fx, double* fy, double* fz, size_t n) {
+    if (!L || !fx || !fy || !fz) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (!L->force) return drp_fail(DRP_ERR_STATE, "no force field (force_kind is not FIELD or SPECTRAL)");
+    if (n != (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != %lld nodes", n, (long long)L->n);
+    double* dst[3] = {fx, fy, fz};
+    for (int k = 0; k < 3; k++)
+        HIP_TRY(hipMemcpyAsync(dst[k], L->force + k * L->n, n * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
     return DRP_OK;
 }

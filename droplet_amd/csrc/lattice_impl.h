@@ -24,7 +24,9 @@ struct drp_lattice {
     void* mac[2] = {nullptr, nullptr};  // phi, p, u1, u2, u3 (5n)
     uint64_t* range = nullptr;          // 3 rotating (max, min) slots of the phi range
     double* tau = nullptr;              // LES relaxation-time field
-    double* force = nullptr;            // FORCE_FIELD: fx | fy | fz
+    double* force = nullptr;            // FORCE_FIELD / SPECTRAL: fx | fy | fz
+    double* spec = nullptr;             // SPECTRAL: transform work space and twiddle tables
+    drp::SpectralWork specw{};
     double* tg = nullptr;               // Taylor-Green sin/cos tables
     double* partial = nullptr;          // diagnostics partial sums
     int64_t partial_cap = 0;
@@ -38,6 +40,8 @@ int lattice_alloc(drp_lattice* L);
 int lattice_create(const drp_config* cfg, hipStream_t stream, drp_lattice** out);
 void lattice_free(drp_lattice* L);
 drp::StepArgs make_step_args(drp_lattice* L, int64_t nstep);
+// SPECTRAL: recompute the force field from the current velocity (FP64 or FP32 fields)
+int spectral_update(drp_lattice* L, int64_t nstep);
 int reset_range(drp_lattice* L);
 
 // deformation.cpp

@@ -1,0 +1,282 @@
+// spectral.hip — linear forcing of the low Fourier shells for forced isotropic
+// turbulence (program/lbm_drop_iht_xy.f90:722-841, externalforce).
+//
+// The reference transforms u with a full 3-D r2c FFT (2decomp + FFTW), keeps the
+// modes with 1 <= |k| < kc (integer mode numbers, kc = 3), computes their energy
+//   E_low = 1/2 sum_{k in S} |u_hat_k|^2,   u_hat = FFT(u) / N^3,
+// and transforms F_hat_k = (eps / (2 E_low)) u_hat_k (zero off S) back, so that
+// the injected power <F . u> is eps by construction.
+//
+// Only the few modes of S (|k_a| <= K = kc - 1 per axis) enter, so here the
+// transform is a truncated, separable DFT instead of three full FFTs:
+//   rows_fwd  A(kx, y, z)   = sum_x u e^{-2 pi i kx x/nx}        kx = 0..K (u real:
+//                                                               -kx is conj(kx))
+//   cols_fwd  B(kx, ky, z)  = sum_y A e^{-2 pi i ky y/ny}        ky = -K..K
+//   modes     u_hat(k)      = sum_z B e^{-2 pi i kz z/nz} / N^3, E_low and
+//             the coefficients C(k) = eps / (2 E_low) u_hat(k) on S
+//   cols_inv  G(kx, ky, z)  = sum_kz C e^{+2 pi i kz z/nz}
+//   rows_inv  H(kx, y, z)   = sum_ky G e^{+2 pi i ky y/ny}
+//   force     F(x, y, z)    = Re H(0) + 2 sum_{kx>0} Re(H(kx) e^{+2 pi i kx x/nx})
+// The first and last passes touch every node once (read u, write F); the others
+// work on N^2 rows or less.  E_low = 0 gives F = 0 (the reference would divide
+// by zero).
+#include "kernels.h"
+
+namespace drp {
+
+namespace {
+
+constexpr int SF_BLOCK = 256;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// e^{s 2 pi i m / n} from the table tw[j] = (cos 2 pi j/n, sin 2 pi j/n), s = +-1
+__device__ __forceinline__ double2 twiddle(const double2* tw, long m, int n, int s) {
+    long j = m % n;
+    if (j < 0) j += n;
+    const double2 t = tw[j];
+    return make_double2(t.x, s * t.y);
+}
+
+// pass 1: one wavefront per (y, z) row
+template <int K, typename T>
+__global__ __launch_bounds__(SF_BLOCK) void sf_rows_fwd(int nx, int ny, int nz, const T* __restrict__ u1,
+                                                        const T* __restrict__ u2, const T* __restrict__ u3,
+                                                        const double2* __restrict__ twx, double2* __restrict__ A) {
+    const int lane = threadIdx.x & 63;
+    const long row = (long)blockIdx.x * (SF_BLOCK / 64) + (threadIdx.x >> 6);
+    if (row >= (long)ny * nz) return;  // uniform per wavefront
+    const T* u[3] = {u1, u2, u3};
+    double re[3][K + 1], im[3][K + 1];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k <= K; k++) re[c][k] = im[c][k] = 0.0;
+    for (int x = lane; x < nx; x += 64) {
+        const long s = row * nx + x;
+        const double v[3] = {(double)u[0][s], (double)u[1][s], (double)u[2][s]};
+#pragma unroll
+        for (int k = 0; k <= K; k++) {
+            const double2 e = twiddle(twx, (long)k * x, nx, -1);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                re[c][k] += v[c] * e.x;
+                im[c][k] += v[c] * e.y;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int k = 0; k <= K; k++)
+            for (int off = 32; off > 0; off >>= 1) {
+                re[c][k] += __shfl_xor(re[c][k], off, 64);
+                im[c][k] += __shfl_xor(im[c][k], off, 64);
+            }
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int k = 0; k <= K; k++) A[(row * 3 + c) * (K + 1) + k] = make_double2(re[c][k], im[c][k]);
+}
+
+// pass 2: one workgroup per z; one thread per (c, kx, ky)
+template <int K>
+__global__ __launch_bounds__(SF_BLOCK) void sf_cols_fwd(int ny, int nz, const double2* __restrict__ A,
+                                                        const double2* __restrict__ twy, double2* __restrict__ B) {
+    constexpr int NK = 2 * K + 1, NOUT = 3 * (K + 1) * NK;
+    const int z = blockIdx.x;
+    for (int t = threadIdx.x; t < NOUT; t += SF_BLOCK) {
+        const int ky = t % NK - K, kx = (t / NK) % (K + 1), c = t / (NK * (K + 1));
+        double2 s = make_double2(0.0, 0.0);
+        for (int y = 0; y < ny; y++) {
+            const double2 a = A[(((long)z * ny + y) * 3 + c) * (K + 1) + kx];
+            const double2 p = cmul(a, twiddle(twy, (long)ky * y, ny, -1));
+            s.x += p.x;
+            s.y += p.y;
+        }
+        B[(long)z * NOUT + t] = s;
+    }
+}
+
+// pass 3: one workgroup; u_hat on the (K+1) x NK x NK modes, E_low, coefficients
+template <int K>
+__global__ __launch_bounds__(SF_BLOCK) void sf_modes(int nx, int ny, int nz, int kc, double eps,
+                                                     const double2* __restrict__ B, const double2* __restrict__ twz,
+                                                     double2* __restrict__ C, double* __restrict__ elow_out) {
+    constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK, NC = NB * NK;
+    __shared__ double2 s_u[NC];
+    __shared__ double s_e[SF_BLOCK / 64];
+    const double inv = 1.0 / ((double)nx * ny * nz);
+    double e = 0.0;
+    for (int t = threadIdx.x; t < NC; t += SF_BLOCK) {
+        const int kz = t % NK - K, b = t / NK;          // b = (c, kx, ky) index of pass 2
+        const int ky = b % NK - K, kx = (b / NK) % (K + 1);
+        double2 s = make_double2(0.0, 0.0);
+        for (int z = 0; z < nz; z++) {
+            const double2 p = cmul(B[(long)z * NB + b], twiddle(twz, (long)kz * z, nz, -1));
+            s.x += p.x;
+            s.y += p.y;
+        }
+        s.x *= inv;
+        s.y *= inv;
+        s_u[t] = s;
+        const int k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 >= 1 && k2 < kc * kc) e += (kx == 0 ? 0.5 : 1.0) * (s.x * s.x + s.y * s.y);
+    }
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+    if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = e;
+    __syncthreads();
+    double elow = 0.0;
+    for (int w = 0; w < SF_BLOCK / 64; w++) elow += s_e[w];
+    const double alpha = elow > 0.0 ? 0.5 * eps / elow : 0.0;
+    for (int t = threadIdx.x; t < NC; t += SF_BLOCK) {
+        const int kz = t % NK - K, b = t / NK;
+        const int ky = b % NK - K, kx = (b / NK) % (K + 1);
+        const int k2 = kx * kx + ky * ky + kz * kz;
+        const bool in = k2 >= 1 && k2 < kc * kc;
+        C[t] = in ? make_double2(alpha * s_u[t].x, alpha * s_u[t].y) : make_double2(0.0, 0.0);
+    }
+    if (threadIdx.x == 0) *elow_out = elow;
+}
+
+// pass 4: G(c, kx, ky, z); one thread per output
+template <int K>
+__global__ __launch_bounds__(SF_BLOCK) void sf_cols_inv(int nz, const double2* __restrict__ C,
+                                                        const double2* __restrict__ twz, double2* __restrict__ G) {
+    constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK;
+    const long t = blockIdx.x * (long)SF_BLOCK + threadIdx.x;
+    if (t >= (long)nz * NB) return;
+    const int b = (int)(t % NB), z = (int)(t / NB);
+    double2 s = make_double2(0.0, 0.0);
+    for (int kz = -K; kz <= K; kz++) {
+        const double2 p = cmul(C[b * NK + (kz + K)], twiddle(twz, (long)kz * z, nz, +1));
+        s.x += p.x;
+        s.y += p.y;
+    }
+    G[t] = s;  // [z][c][kx][ky]
+}
+
+// pass 5: H(c, kx, y, z); one thread per output
+template <int K>
+__global__ __launch_bounds__(SF_BLOCK) void sf_rows_inv(int ny, int nz, const double2* __restrict__ G,
+                                                        const double2* __restrict__ twy, double2* __restrict__ H) {
+    constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK, NH = 3 * (K + 1);
+    const long t = blockIdx.x * (long)SF_BLOCK + threadIdx.x;
+    if (t >= (long)ny * nz * NH) return;
+    const int ck = (int)(t % NH);                  // c * (K+1) + kx
+    const long row = t / NH;
+    const int y = (int)(row % ny), z = (int)(row / ny);
+    double2 s = make_double2(0.0, 0.0);
+    for (int ky = -K; ky <= K; ky++) {
+        const double2 p = cmul(G[(long)z * NB + ck * NK + (ky + K)], twiddle(twy, (long)ky * y, ny, +1));
+        s.x += p.x;
+        s.y += p.y;
+    }
+    H[t] = s;  // [row][c][kx]
+}
+
+// pass 6: the force field at every node
+template <int K>
+__global__ __launch_bounds__(SF_BLOCK) void sf_force(int nx, int ny, int nz, const double2* __restrict__ H,
+                                                     const double2* __restrict__ twx, double* __restrict__ fx,
+                                                     double* __restrict__ fy, double* __restrict__ fz) {
+    const long n = (long)nx * ny * nz;
+    const long s = blockIdx.x * (long)SF_BLOCK + threadIdx.x;
+    if (s >= n) return;
+    const int x = (int)(s % nx);
+    const long row = s / nx;
+    double f[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const double2* h = H + (row * 3 + c) * (K + 1);
+        double v = h[0].x;
+#pragma unroll
+        for (int k = 1; k <= K; k++) {
+            const double2 e = twiddle(twx, (long)k * x, nx, +1);
+            v += 2.0 * (h[k].x * e.x - h[k].y * e.y);
+        }
+        f[c] = v;
+    }
+    fx[s] = f[0];
+    fy[s] = f[1];
+    fz[s] = f[2];
+}
+
+template <int K, typename T>
+hipError_t spectral_k(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const T* u1, const T* u2,
+                      const T* u3, double* fx, double* fy, double* fz, hipStream_t st) {
+    constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK;
+    const long rows = (long)ny * nz, n = rows * nx;
+    hipLaunchKernelGGL((sf_rows_fwd<K, T>), dim3((rows + 3) / 4), dim3(SF_BLOCK), 0, st, nx, ny, nz, u1, u2, u3,
+                       w.twx, w.A);
+    hipLaunchKernelGGL((sf_cols_fwd<K>), dim3(nz), dim3(SF_BLOCK), 0, st, ny, nz, w.A, w.twy, w.B);
+    hipLaunchKernelGGL((sf_modes<K>), dim3(1), dim3(SF_BLOCK), 0, st, nx, ny, nz, kc, eps, w.B, w.twz, w.C, w.elow);
+    hipLaunchKernelGGL((sf_cols_inv<K>), dim3((nz * NB + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st, nz, w.C,
+                       w.twz, w.G);
+    hipLaunchKernelGGL((sf_rows_inv<K>), dim3((rows * 3 * (K + 1) + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st,
+                       ny, nz, w.G, w.twy, w.A);  // H reuses A
+    hipLaunchKernelGGL((sf_force<K>), dim3((n + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st, nx, ny, nz, w.A,
+                       w.twx, fx, fy, fz);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t spectral_dispatch(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const T* u1,
+                             const T* u2, const T* u3, double* fx, double* fy, double* fz, hipStream_t st) {
+    switch (kc - 1) {
+        case 1: return spectral_k<1, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+        case 2: return spectral_k<2, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+        case 3: return spectral_k<3, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+        case 4: return spectral_k<4, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+size_t spectral_work_doubles(int nx, int ny, int nz, int kc) {
+    const size_t K = (size_t)(kc - 1), NK = 2 * K + 1, NB = 3 * (K + 1) * NK;
+    const size_t rows = (size_t)ny * nz;
+    const size_t cplx = rows * 3 * (K + 1)      // A / H
+                        + (size_t)nz * NB        // B
+                        + NB * NK                // C
+                        + (size_t)nz * NB        // G
+                        + (size_t)nx + ny + nz;  // twiddle tables
+    return 2 * cplx + 1;                         // + E_low
+}
+
+void spectral_work_layout(SpectralWork& w, double* base, int nx, int ny, int nz, int kc) {
+    const size_t K = (size_t)(kc - 1), NK = 2 * K + 1, NB = 3 * (K + 1) * NK;
+    double2* p = reinterpret_cast<double2*>(base);
+    w.A = p;
+    p += (size_t)ny * nz * 3 * (K + 1);
+    w.B = p;
+    p += (size_t)nz * NB;
+    w.C = p;
+    p += NB * NK;
+    w.G = p;
+    p += (size_t)nz * NB;
+    w.twx = p;
+    p += nx;
+    w.twy = p;
+    p += ny;
+    w.twz = p;
+    p += nz;
+    w.elow = reinterpret_cast<double*>(p);
+}
+
+hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const double* u1,
+                                 const double* u2, const double* u3, double* fx, double* fy, double* fz,
+                                 hipStream_t st) {
+    return spectral_dispatch<double>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+}
+
+hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const float* u1,
+                                 const float* u2, const float* u3, double* fx, double* fy, double* fz,
+                                 hipStream_t st) {
+    return spectral_dispatch<float>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+}
+
+}  // namespace drp

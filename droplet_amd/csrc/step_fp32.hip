@@ -9,8 +9,8 @@
 // relative precision.  Sum_q fr_q = phi1 and Sum_q E_q (1, c) = (1, 0), so the
 // moments of the deviations are the deviations of the moments.
 //
-// Arithmetic: equilibria and collision in FP32, written directly in deviation
-// form (no difference of two large numbers):
+// Arithmetic: equilibria and collision in FP32 with fused multiply-adds,
+// written directly in deviation form (no difference of two large numbers):
 //   p0(phi) - p0(phi1) = phit [T / ((1 - b phi)(1 - b phi1)) - a (phi + phi1)]
 //   ft_eq = H phit + F At + 3 E phi (c.u) + E kf gt
 //   gt_eq = E (3 pt + 3 c.u + 4.5 (c.u)^2 - 1.5 u^2) + E kg gt
@@ -53,9 +53,16 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     constexpr int HX1 = TX + 2, HY1 = TY + 2, HZ1 = TZ + 2, H1 = HX1 * HY1 * HZ1;
     static_assert(NN % 64 == 0, "a role must own whole wavefronts");
 
+    // Per halo node the stencil phase leaves four float4 records, so that the
+    // collision phase needs two 16-byte LDS reads and about ten FMAs per direction:
+    //   fa = (At, 3 phi u1, 3 phi u2, 3 phi u3)       phi-population equilibrium
+    //   ga = (3 pt - 1.5 |u|^2, u1, u2, u3)           momentum equilibrium
+    //   ka = (K0, K1, K2, -)   K_k = 4.5 g_k^2 - 1.5 |g|^2     Korteweg, axis c
+    //   pa = (9 g0 g1, 9 g0 g2, 9 g1 g2, -)                      Korteweg, diagonal c
+    // For a diagonal c (c_a^2 = 1) the contraction 4.5 (c.g)^2 - 1.5 |g|^2 |c|^2
+    // reduces to 9 (cx cy g0 g1 + cx cz g0 g2 + cy cz g1 g2).
     __shared__ float s_phi[H2];  // phit
-    __shared__ float s_p[H1], s_u1[H1], s_u2[H1], s_u3[H1];
-    __shared__ float s_A[H1], s_g0[H1], s_g1[H1], s_g2[H1];
+    __shared__ float4 s_fa[H1], s_ga[H1], s_ka[H1], s_pa[H1];
     __shared__ float s_rtau[TAUMODE == 0 ? 1 : H1];  // 1 / tau_g
     __shared__ double s_red[2][NT / 64];
 
@@ -95,7 +102,7 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         });
     }
 
-    // phase 1: stage fields
+    // phase 1: stage fields (p and u into ga; p is replaced in phase 2)
     const bool wm = a.wrap_mod != 0;
     for (int h = tid; h < H2; h += NT) {
         const int hx = h % HX2, hy = (h / HX2) % HY2, hz = h / (HX2 * HY2);
@@ -106,15 +113,13 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
         const int gx = wrap32(x0 + hx - 1, nx, wm), gy = wrap32(y0 + hy - 1, ny, wm), gz = wrap32(z0 + hz - 1, nz, wm);
         const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-        s_p[h] = p_in[g];
-        s_u1[h] = u1_in[g];
-        s_u2[h] = u2_in[g];
-        s_u3[h] = u3_in[g];
+        s_ga[h] = make_float4(p_in[g], u1_in[g], u2_in[g], u3_in[g]);
         if constexpr (TAUMODE == 2) s_rtau[h] = 1.0f / a.tau_field[g];
     }
     __syncthreads();
 
-    // phase 2: FP64-accumulated Laplacian and gradient, deviation F-bracket, 1/tau
+    // phase 2: FP64-accumulated Laplacian and gradient, deviation F-bracket, the
+    // equilibrium records, 1/tau
     const double rmax = from_ordered_bits(a.range[2 * a.slot_read + 0]);
     const double rmin = from_ordered_bits(a.range[2 * a.slot_read + 1]);
     const float range = (float)(rmax - rmin), rminf = (float)(rmin - a.phi1);
@@ -135,11 +140,18 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         const float lapf = (float)(lap * 0.2), gf0 = (float)(g0 * 0.1), gf1 = (float)(g1 * 0.1),
                     gf2 = (float)(g2 * 0.1);
         const float ph = a.phi1f + pt;
+        const float gsq = __builtin_fmaf(gf0, gf0, __builtin_fmaf(gf1, gf1, gf2 * gf2));
         const float dp0 = pt * (a.Tf / ((1.0f - a.bf * ph) * a.one_m_bphi1) - a.af * (ph + a.phi1f));
-        s_A[h] = dp0 - a.kff * ph * lapf - a.kf6f * (gf0 * gf0 + gf1 * gf1 + gf2 * gf2);
-        s_g0[h] = gf0;
-        s_g1[h] = gf1;
-        s_g2[h] = gf2;
+        const float At = dp0 - a.kff * ph * lapf - a.kf6f * gsq;
+        const float4 pu = s_ga[h];
+        const float usq = __builtin_fmaf(pu.y, pu.y, __builtin_fmaf(pu.z, pu.z, pu.w * pu.w));
+        const float ph3 = 3.0f * ph;
+        s_fa[h] = make_float4(At, ph3 * pu.y, ph3 * pu.z, ph3 * pu.w);
+        s_ga[h] = make_float4(__builtin_fmaf(-1.5f, usq, 3.0f * pu.x), pu.y, pu.z, pu.w);
+        const float s2h = 1.5f * gsq;
+        s_ka[h] = make_float4(__builtin_fmaf(4.5f * gf0, gf0, -s2h), __builtin_fmaf(4.5f * gf1, gf1, -s2h),
+                              __builtin_fmaf(4.5f * gf2, gf2, -s2h), 0.0f);
+        s_pa[h] = make_float4(9.0f * gf0 * gf1, 9.0f * gf0 * gf2, 9.0f * gf1 * gf2, 0.0f);
         if constexpr (TAUMODE == 1) {
             float frac = (range == 0.0f) ? 0.0f : (pt - rminf) / range;
             s_rtau[h] = 1.0f / (0.5f + 3.0f * (frac * a.dnuf + a.nu1f));
@@ -147,30 +159,47 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     }
     __syncthreads();
 
+    // Korteweg contraction gt_q and c_q . v from the records of the upstream node
+    auto gt_of = [&](auto qc, int hu) -> float {
+        constexpr int q = decltype(qc)::value;
+        using D = Dir<q>;
+        if constexpr (q == 0) {
+            return 0.0f;
+        } else if constexpr (q <= 6) {
+            const float4 k = s_ka[hu];
+            return D::cx != 0 ? k.x : (D::cy != 0 ? k.y : k.z);
+        } else {
+            const float4 pp = s_pa[hu];
+            float t = (D::cx * D::cy > 0) ? pp.x : -pp.x;
+            t = (D::cx * D::cz > 0) ? t + pp.y : t - pp.y;
+            return (D::cy * D::cz > 0) ? t + pp.z : t - pp.z;
+        }
+    };
+
     // phase 3: collide-stream
     double vmax = -__builtin_huge_val(), vmin = __builtin_huge_val();
     if (inside) {
         const int64_t s = x + (int64_t)nx * (y + (int64_t)ny * z);
+        const int hself = (tx + 1) + HX1 * ((ty + 1) + HY1 * (tz + 1));
         if (!mom) {
             double phs = 0.0;
+            const float ekf7 = (float)weightE<7>() * a.kff, ekf1 = (float)weightE<1>() * a.kff;
             sfor32<0, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
                 constexpr float E = (float)weightE<q>(), Fw = (float)weightF<q>();
                 const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
-                const int h2 = (tx + 2 - D::cx) + HX2 * ((ty + 2 - D::cy) + HY2 * (tz + 2 - D::cz));
-                const float pt = s_phi[h2];
-                float fe = Fw * s_A[hu];
+                float fe;
                 if constexpr (q == 0) {
-                    fe += pt;
+                    const int h2 = (tx + 2) + HX2 * ((ty + 2) + HY2 * (tz + 2));
+                    fe = __builtin_fmaf(Fw, s_fa[hself].x, s_phi[h2]);
                 } else {
-                    const float g0 = s_g0[hu], g1 = s_g1[hu], g2 = s_g2[hu];
-                    const float s2h = 1.5f * (g0 * g0 + g1 * g1 + g2 * g2);
-                    const float gt = korteweg<q>(g0, g1, g2, s2h);
-                    const float cu = cdot<q>(s_u1[hu], s_u2[hu], s_u3[hu]);
-                    fe += 3.0f * E * (a.phi1f + pt) * cu + E * a.kff * gt;
+                    const float4 fa = s_fa[hu];
+                    const float cv = cdot<q>(fa.y, fa.z, fa.w);
+                    const float gt = gt_of(qc, hu);
+                    fe = __builtin_fmaf(Fw, fa.x, __builtin_fmaf(E, cv, (q <= 6 ? ekf1 : ekf7) * gt));
                 }
-                const float fv = ps[q] - (ps[q] - fe) * a.rtau_f;
+                const float fv = __builtin_fmaf(fe - ps[q], a.rtau_f, ps[q]);
                 phs += (double)fv;
                 __builtin_nontemporal_store(fv, &a.f_out[q * N + s]);
                 __builtin_amdgcn_sched_barrier(0);
@@ -178,25 +207,23 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
             __builtin_nontemporal_store((float)phs, &a.mac_out[s]);
             vmax = vmin = phs + a.phi1;
         } else {
+            const float ekg7 = (float)weightE<7>() * a.kgf, ekg1 = (float)weightE<1>() * a.kgf;
             sfor32<0, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
                 constexpr float E = (float)weightE<q>();
                 const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
-                const float u1 = s_u1[hu], u2 = s_u2[hu], u3 = s_u3[hu];
-                const float usq = u1 * u1 + u2 * u2 + u3 * u3;
                 float ge;
                 if constexpr (q == 0) {
-                    ge = E * (3.0f * s_p[hu] - 1.5f * usq);
+                    ge = E * s_ga[hself].x;
                 } else {
-                    const float g0 = s_g0[hu], g1 = s_g1[hu], g2 = s_g2[hu];
-                    const float s2h = 1.5f * (g0 * g0 + g1 * g1 + g2 * g2);
-                    const float gt = korteweg<q>(g0, g1, g2, s2h);
-                    const float cu = cdot<q>(u1, u2, u3);
-                    ge = E * (3.0f * s_p[hu] + 3.0f * cu + 4.5f * cu * cu - 1.5f * usq) + E * a.kgf * gt;
+                    const float4 ga = s_ga[hu];
+                    const float cu = cdot<q>(ga.y, ga.z, ga.w);
+                    const float t = __builtin_fmaf(cu, __builtin_fmaf(4.5f, cu, 3.0f), ga.x);
+                    ge = __builtin_fmaf(E, t, (q <= 6 ? ekg1 : ekg7) * gt_of(qc, hu));
                 }
                 const float rt = (TAUMODE == 0) ? a.rtau_const : s_rtau[hu];
-                float gv = ps[q] - (ps[q] - ge) * rt;
+                float gv = __builtin_fmaf(ge - ps[q], rt, ps[q]);
                 if constexpr (FORCE != FORCE_NONE && q != 0) {
                     float Fx, Fy, Fz;
                     if constexpr (FORCE == FORCE_TAYLOR_GREEN) {

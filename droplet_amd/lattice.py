@@ -55,6 +55,7 @@ def lib():
         L.drp_inject_momentum.argtypes = [vp, dp, sz]
         L.drp_set_force.argtypes = [vp, dp, dp, dp, sz]
         L.drp_reduce.argtypes = [vp, C.POINTER(Stats)]
+        L.drp_get_force.argtypes = [vp, dp, dp, dp, sz]
         L.drp_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
         L.drp_time_steps.argtypes = [vp, C.c_int, dp]
         L.drp_tile_shape_count.argtypes = []
@@ -187,6 +188,12 @@ class Lattice:
     def set_force(self, fx, fy, fz):
         fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (fx, fy, fz)]
         _check(lib().drp_set_force(self._h, *[_dp(a) for a in fs], self.n), "drp_set_force")
+
+    def force(self):
+        """(fx, fy, fz) of the body-force field the last step used."""
+        out = [np.empty(self.shape, np.float64) for _ in range(3)]
+        _check(lib().drp_get_force(self._h, *[_dp(a) for a in out], self.n), "drp_get_force")
+        return tuple(out)
 
     # -- analysis hooks
     def reduce(self) -> dict:

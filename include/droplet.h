@@ -56,6 +56,9 @@ extern "C" {
 #define DRP_FORCE_NONE 0
 #define DRP_FORCE_TAYLOR_GREEN 1 /* externalforce, program/lbm_drop_xy.f90:668-693 */
 #define DRP_FORCE_FIELD 2        /* user-supplied static field (drp_set_force)      */
+#define DRP_FORCE_SPECTRAL 3     /* linear forcing of the shells 1 <= |k| < force_k with
+                                    injection rate force_amp, recomputed from u every step
+                                    (externalforce, program/lbm_drop_iht_xy.f90:722-841) */
 
 /* macroscopic fields for drp_get_macro / drp_set_macro */
 #define DRP_FIELD_PHI 0
@@ -83,8 +86,8 @@ typedef struct drp_config {
     double wall_u;        /* wall speed U (ref uw)                                     */
     int32_t force_kind;   /* DRP_FORCE_*                                               */
     int32_t force_start;  /* first step whose collision includes the force (ref step_input) */
-    double force_amp;     /* Taylor-Green amplitude 2 pi^2 U^2/(D_vortex Re)           */
-    double force_k;       /* Taylor-Green wavenumber pi/D_vortex (ref kw)              */
+    double force_amp;     /* Taylor-Green amplitude 2 pi^2 U^2/(D_vortex Re); spectral: epsilon */
+    double force_k;       /* Taylor-Green wavenumber pi/D_vortex (ref kw); spectral: kc (2..5) */
     double les_cs;        /* Smagorinsky constant Cs (0 = no LES)                      */
     double les_delta;     /* filter width Delta                                        */
     int32_t reserved[8];  /* must be zero                                              */
@@ -232,6 +235,11 @@ int drp_tape_backward(drp_tape* t, const double w[2], double* loss, double grad[
 /* Device bytes held by the tape's checkpoints and work buffers. */
 int64_t drp_tape_bytes(const drp_tape* t);
 void drp_tape_destroy(drp_tape* t);
+
+/* Current body-force field (DRP_FORCE_FIELD / DRP_FORCE_SPECTRAL: the field the last
+ * step used) to host, three arrays of nx*ny*nz doubles.  No reference counterpart
+ * (forcex/forcey/forcez are module arrays there, program/lbm_drop_iht_xy.f90:722-841). */
+int drp_get_force(drp_lattice* lat, double* fx, double* fy, double* fz, size_t n);
 
 /* Achievable HBM bandwidth of a plain stream copy of `bytes` (read + write counted),
  * averaged over reps launches on a private stream. */

@@ -269,6 +269,68 @@ static void taylor_green_force(ora_lattice* L) {
             }
 }
 
+/* Linear forcing of the low shells, externalforce of program/lbm_drop_iht_xy.f90:722-841:
+   u_hat = DFT(u) / N^3 (:894-899), E_low = 1/2 sum_{1 <= |k| < kc} |u_hat|^2 (:773-790, the
+   half-spectrum sum with weights min(2, k3+1) is the full-spectrum sum), F_hat =
+   eps / (2 E_low) u_hat on those shells (:796-814), F = inverse DFT (:818-821).  Computed
+   here as a direct DFT over the modes |k_a| <= kc - 1 with the trigonometric functions
+   evaluated per term (FFTW, which the reference links, is not available: parity of this
+   routine is pinned by numpy's FFT and the injection invariant <F.u> = eps instead).
+   Returns E_low; F = 0 when E_low = 0. */
+double ora_spectral_force_fields(const ora_lattice* L, const double* u1, const double* u2, const double* u3,
+                                 double* fx, double* fy, double* fz) {
+    const drp_config* c = &L->c;
+    const int kc = (int)c->force_k, K = kc - 1, NK = 2 * K + 1, NM = NK * NK * NK;
+    const double pi = acos(-1.0), inv = 1.0 / (double)L->n;
+    const double* u[3] = {u1, u2, u3};
+    double* f[3] = {fx, fy, fz};
+    double* re = calloc((size_t)3 * NM, sizeof(double));
+    double* im = calloc((size_t)3 * NM, sizeof(double));
+#pragma omp parallel for num_threads(L->threads) schedule(dynamic)
+    for (int m = 0; m < NM; m++) {
+        const int kx = m % NK - K, ky = (m / NK) % NK - K, kz = m / (NK * NK) - K;
+        const int k2 = kx * kx + ky * ky + kz * kz;
+        if (k2 < 1 || k2 >= kc * kc) continue;
+        double sr[3] = {0, 0, 0}, si[3] = {0, 0, 0};
+        for (int z = 0; z < c->nz; z++)
+            for (int y = 0; y < c->ny; y++)
+                for (int x = 0; x < c->nx; x++) {
+                    const double th = 2.0 * pi * ((double)kx * x / c->nx + (double)ky * y / c->ny + (double)kz * z / c->nz);
+                    const double cs = cos(th), sn = sin(th);
+                    const int64_t s = idx(L, x, y, z);
+                    for (int a = 0; a < 3; a++) {
+                        sr[a] += u[a][s] * cs;
+                        si[a] -= u[a][s] * sn;
+                    }
+                }
+        for (int a = 0; a < 3; a++) {
+            re[a * NM + m] = sr[a] * inv;
+            im[a * NM + m] = si[a] * inv;
+        }
+    }
+    double elow = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int m = 0; m < NM; m++) elow += 0.5 * (re[a * NM + m] * re[a * NM + m] + im[a * NM + m] * im[a * NM + m]);
+    const double alpha = elow > 0.0 ? 0.5 * c->force_amp / elow : 0.0;
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+    for (int64_t s = 0; s < L->n; s++) {
+        const int x = (int)(s % c->nx), y = (int)((s / c->nx) % c->ny), z = (int)(s / ((int64_t)c->nx * c->ny));
+        double v[3] = {0, 0, 0};
+        for (int m = 0; m < NM; m++) {
+            const int kx = m % NK - K, ky = (m / NK) % NK - K, kz = m / (NK * NK) - K;
+            const int k2 = kx * kx + ky * ky + kz * kz;
+            if (k2 < 1 || k2 >= kc * kc) continue;
+            const double th = 2.0 * pi * ((double)kx * x / c->nx + (double)ky * y / c->ny + (double)kz * z / c->nz);
+            const double cs = cos(th), sn = sin(th);
+            for (int a = 0; a < 3; a++) v[a] += re[a * NM + m] * cs - im[a * NM + m] * sn;
+        }
+        for (int a = 0; a < 3; a++) f[a][s] = alpha * v[a];
+    }
+    free(re);
+    free(im);
+    return elow;
+}
+
 /* ---------------------------------------------------------------- initialisation */
 
 static void init_equilibrium_populations(ora_lattice* L) {
@@ -374,6 +436,9 @@ int ora_step(ora_lattice* L, int nsteps) {
             force_on = 1;
         } else if (c->force_kind == DRP_FORCE_FIELD && nstep >= c->force_start) {
             force_on = 1;
+        } else if (c->force_kind == DRP_FORCE_SPECTRAL && nstep >= c->force_start) {
+            ora_spectral_force_fields(L, L->u1, L->u2, L->u3, L->fx, L->fy, L->fz);
+            force_on = 1;
         }
         compute_equilibria(L);      /* various_cal, gphi_cal, equilibrium_cal (+ tau of physics) */
         stream_collide(L, force_on); /* fg_cal */
@@ -433,6 +498,14 @@ int ora_set_force(ora_lattice* L, const double* fx, const double* fy, const doub
     memcpy(L->fx, fx, b);
     memcpy(L->fy, fy, b);
     memcpy(L->fz, fz, b);
+    return 0;
+}
+
+int ora_get_force(const ora_lattice* L, double* fx, double* fy, double* fz) {
+    size_t b = sizeof(double) * (size_t)L->n;
+    memcpy(fx, L->fx, b);
+    memcpy(fy, L->fy, b);
+    memcpy(fz, L->fz, b);
     return 0;
 }
 

@@ -64,6 +64,9 @@ def _bind(L):
     L.ora_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
     L.ora_stencils.argtypes = [vp, dp, dp, dp]
     L.ora_equilibria.argtypes = [vp, dp, dp, dp]
+    L.ora_get_force.argtypes = [vp, dp, dp, dp]
+    L.ora_spectral_force_fields.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.ora_spectral_force_fields.restype = C.c_double
     return L
 
 
@@ -135,6 +138,18 @@ class OracleLattice:
     def set_force(self, fx, fy, fz):
         fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (fx, fy, fz)]
         self._L.ora_set_force(self._h, *[_dp(a) for a in fs])
+
+    def force(self):
+        out = [np.empty(self.shape, np.float64) for _ in range(3)]
+        self._L.ora_get_force(self._h, *[_dp(a) for a in out])
+        return out
+
+    def spectral_force(self, u1, u2, u3):
+        """(fx, fy, fz, E_low) of the low-shell forcing for the given velocity (config's force_k, force_amp)."""
+        us = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (u1, u2, u3)]
+        out = [np.empty(self.shape, np.float64) for _ in range(3)]
+        elow = self._L.ora_spectral_force_fields(self._h, *[_dp(a) for a in us], *[_dp(a) for a in out])
+        return out[0], out[1], out[2], elow
 
     def phi_range(self):
         mn, mx = C.c_double(), C.c_double()
