@@ -12,13 +12,14 @@
 //   rows_fwd  A(kx, y, z)   = sum_x u e^{-2 pi i kx x/nx}        kx = 0..K (u real:
 //                                                               -kx is conj(kx))
 //   cols_fwd  B(kx, ky, z)  = sum_y A e^{-2 pi i ky y/ny}        ky = -K..K
-//   modes     u_hat(k)      = sum_z B e^{-2 pi i kz z/nz} / N^3, E_low and
-//             the coefficients C(k) = eps / (2 E_low) u_hat(k) on S
+//   zsum      u_hat(k)      = sum_z B e^{-2 pi i kz z/nz} / N^3
+//   coef      E_low and the coefficients C(k) = eps / (2 E_low) u_hat(k) on S
 //   cols_inv  G(kx, ky, z)  = sum_kz C e^{+2 pi i kz z/nz}
 //   rows_inv  H(kx, y, z)   = sum_ky G e^{+2 pi i ky y/ny}
 //   force     F(x, y, z)    = Re H(0) + 2 sum_{kx>0} Re(H(kx) e^{+2 pi i kx x/nx})
 // The first and last passes touch every node once (read u, write F); the others
-// work on N^2 rows or less.  E_low = 0 gives F = 0 (the reference would divide
+// work on N^2 rows or less, one wavefront per output with the lanes over the
+// summed axis.  E_low = 0 gives F = 0 (the reference would divide
 // by zero).
 #include "kernels.h"
 
@@ -81,47 +82,65 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_rows_fwd(int nx, int ny, int nz, 
             for (int k = 0; k <= K; k++) A[(row * 3 + c) * (K + 1) + k] = make_double2(re[c][k], im[c][k]);
 }
 
-// pass 2: one workgroup per z; one thread per (c, kx, ky)
+// pass 2: one wavefront per (z, c, kx, ky), lanes over y
 template <int K>
 __global__ __launch_bounds__(SF_BLOCK) void sf_cols_fwd(int ny, int nz, const double2* __restrict__ A,
                                                         const double2* __restrict__ twy, double2* __restrict__ B) {
-    constexpr int NK = 2 * K + 1, NOUT = 3 * (K + 1) * NK;
-    const int z = blockIdx.x;
-    for (int t = threadIdx.x; t < NOUT; t += SF_BLOCK) {
-        const int ky = t % NK - K, kx = (t / NK) % (K + 1), c = t / (NK * (K + 1));
-        double2 s = make_double2(0.0, 0.0);
-        for (int y = 0; y < ny; y++) {
-            const double2 a = A[(((long)z * ny + y) * 3 + c) * (K + 1) + kx];
-            const double2 p = cmul(a, twiddle(twy, (long)ky * y, ny, -1));
-            s.x += p.x;
-            s.y += p.y;
-        }
-        B[(long)z * NOUT + t] = s;
+    constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK;
+    const int lane = threadIdx.x & 63;
+    const long w = (long)blockIdx.x * (SF_BLOCK / 64) + (threadIdx.x >> 6);
+    if (w >= (long)nz * NB) return;  // uniform per wavefront
+    const int b = (int)(w % NB), z = (int)(w / NB);
+    const int ky = b % NK - K, kx = (b / NK) % (K + 1), c = b / (NK * (K + 1));
+    double sr = 0.0, si = 0.0;
+    for (int y = lane; y < ny; y += 64) {
+        const double2 p = cmul(A[(((long)z * ny + y) * 3 + c) * (K + 1) + kx], twiddle(twy, (long)ky * y, ny, -1));
+        sr += p.x;
+        si += p.y;
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+    }
+    if (lane == 0) B[w] = make_double2(sr, si);  // [z][c][kx][ky]
 }
 
-// pass 3: one workgroup; u_hat on the (K+1) x NK x NK modes, E_low, coefficients
+// pass 3a: one wavefront per mode (c, kx, ky, kz), lanes over z: u_hat into U
 template <int K>
-__global__ __launch_bounds__(SF_BLOCK) void sf_modes(int nx, int ny, int nz, int kc, double eps,
-                                                     const double2* __restrict__ B, const double2* __restrict__ twz,
-                                                     double2* __restrict__ C, double* __restrict__ elow_out) {
+__global__ __launch_bounds__(SF_BLOCK) void sf_zsum(int nx, int ny, int nz, const double2* __restrict__ B,
+                                                    const double2* __restrict__ twz, double2* __restrict__ U) {
+    constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK, NC = NB * NK;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * (SF_BLOCK / 64) + (threadIdx.x >> 6);
+    if (t >= NC) return;  // uniform per wavefront
+    const int kz = t % NK - K, b = t / NK;
+    double sr = 0.0, si = 0.0;
+    for (int z = lane; z < nz; z += 64) {
+        const double2 p = cmul(B[(long)z * NB + b], twiddle(twz, (long)kz * z, nz, -1));
+        sr += p.x;
+        si += p.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+    }
+    const double inv = 1.0 / ((double)nx * ny * nz);
+    if (lane == 0) U[t] = make_double2(sr * inv, si * inv);
+}
+
+// pass 3b: one workgroup; E_low of the forced shells and the coefficients, in place
+template <int K>
+__global__ __launch_bounds__(SF_BLOCK) void sf_coef(int kc, double eps, double2* __restrict__ C,
+                                                    double* __restrict__ elow_out) {
     constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK, NC = NB * NK;
     __shared__ double2 s_u[NC];
     __shared__ double s_e[SF_BLOCK / 64];
-    const double inv = 1.0 / ((double)nx * ny * nz);
     double e = 0.0;
     for (int t = threadIdx.x; t < NC; t += SF_BLOCK) {
-        const int kz = t % NK - K, b = t / NK;          // b = (c, kx, ky) index of pass 2
-        const int ky = b % NK - K, kx = (b / NK) % (K + 1);
-        double2 s = make_double2(0.0, 0.0);
-        for (int z = 0; z < nz; z++) {
-            const double2 p = cmul(B[(long)z * NB + b], twiddle(twz, (long)kz * z, nz, -1));
-            s.x += p.x;
-            s.y += p.y;
-        }
-        s.x *= inv;
-        s.y *= inv;
+        const double2 s = C[t];
         s_u[t] = s;
+        const int kz = t % NK - K, b = t / NK;
+        const int ky = b % NK - K, kx = (b / NK) % (K + 1);
         const int k2 = kx * kx + ky * ky + kz * kz;
         if (k2 >= 1 && k2 < kc * kc) e += (kx == 0 ? 0.5 : 1.0) * (s.x * s.x + s.y * s.y);
     }
@@ -211,8 +230,11 @@ hipError_t spectral_k(const SpectralWork& w, int nx, int ny, int nz, int kc, dou
     const long rows = (long)ny * nz, n = rows * nx;
     hipLaunchKernelGGL((sf_rows_fwd<K, T>), dim3((rows + 3) / 4), dim3(SF_BLOCK), 0, st, nx, ny, nz, u1, u2, u3,
                        w.twx, w.A);
-    hipLaunchKernelGGL((sf_cols_fwd<K>), dim3(nz), dim3(SF_BLOCK), 0, st, ny, nz, w.A, w.twy, w.B);
-    hipLaunchKernelGGL((sf_modes<K>), dim3(1), dim3(SF_BLOCK), 0, st, nx, ny, nz, kc, eps, w.B, w.twz, w.C, w.elow);
+    constexpr int NC = NB * NK, WPB = SF_BLOCK / 64;
+    hipLaunchKernelGGL((sf_cols_fwd<K>), dim3((nz * NB + WPB - 1) / WPB), dim3(SF_BLOCK), 0, st, ny, nz, w.A, w.twy,
+                       w.B);
+    hipLaunchKernelGGL((sf_zsum<K>), dim3((NC + WPB - 1) / WPB), dim3(SF_BLOCK), 0, st, nx, ny, nz, w.B, w.twz, w.C);
+    hipLaunchKernelGGL((sf_coef<K>), dim3(1), dim3(SF_BLOCK), 0, st, kc, eps, w.C, w.elow);
     hipLaunchKernelGGL((sf_cols_inv<K>), dim3((nz * NB + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st, nz, w.C,
                        w.twz, w.G);
     hipLaunchKernelGGL((sf_rows_inv<K>), dim3((rows * 3 * (K + 1) + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st,

@@ -225,19 +225,18 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
                 const float rt = (TAUMODE == 0) ? a.rtau_const : s_rtau[hu];
                 float gv = __builtin_fmaf(ge - ps[q], rt, ps[q]);
                 if constexpr (FORCE != FORCE_NONE && q != 0) {
-                    float Fx, Fy, Fz;
+                    float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f;   // only the components that enter c . F
                     if constexpr (FORCE == FORCE_TAYLOR_GREEN) {
                         const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
-                        Fx = (float)(a.tg_amp * a.sinx[xu] * a.cosz[zu]);
-                        Fy = 0.0f;
-                        Fz = (float)(-(a.tg_amp * a.cosx[xu] * a.sinz[zu]));
+                        if constexpr (D::cx != 0) Fx = (float)(a.tg_amp * a.sinx[xu] * a.cosz[zu]);
+                        if constexpr (D::cz != 0) Fz = (float)(-(a.tg_amp * a.cosx[xu] * a.sinz[zu]));
                     } else {
                         const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
-                        Fx = (float)a.fx[src];
-                        Fy = (float)a.fy[src];
-                        Fz = (float)a.fz[src];
+                        if constexpr (D::cx != 0) Fx = (float)a.fx[src];
+                        if constexpr (D::cy != 0) Fy = (float)a.fy[src];
+                        if constexpr (D::cz != 0) Fz = (float)a.fz[src];
                     }
-                    gv += 3.0f * E * ((float)D::cx * Fx + (float)D::cy * Fy + (float)D::cz * Fz);
+                    gv = __builtin_fmaf(3.0f * E, cdot<q>(Fx, Fy, Fz), gv);
                 }
                 ps[q] = gv;
                 __builtin_amdgcn_sched_barrier(0);

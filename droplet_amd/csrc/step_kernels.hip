@@ -237,20 +237,21 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 const double tau = (TAUMODE == TAU_CONST) ? a.tau_const : s_tau[hu];
                 double gv = ps[q] - (ps[q] - ge) / tau;
                 if constexpr (FORCE != FORCE_NONE && q != 0) {
-                    double Fx, Fy, Fz;
+                    // c . F with the zero-velocity terms dropped, as for c . u: the
+                    // reference's (cx Fx + cy Fy) + cz Fz rounds identically, and
+                    // only the components that enter are loaded
+                    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
                     if constexpr (FORCE == FORCE_TAYLOR_GREEN) {
                         const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
-                        Fx = a.tg_amp * a.sinx[xu] * a.cosz[zu];
-                        Fy = 0.0;
-                        Fz = -(a.tg_amp * a.cosx[xu] * a.sinz[zu]);
+                        if constexpr (D::cx != 0) Fx = a.tg_amp * a.sinx[xu] * a.cosz[zu];
+                        if constexpr (D::cz != 0) Fz = -(a.tg_amp * a.cosx[xu] * a.sinz[zu]);
                     } else {  // static field, read at the upstream node through L2
                         const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
-                        Fx = a.fx[src];
-                        Fy = a.fy[src];
-                        Fz = a.fz[src];
+                        if constexpr (D::cx != 0) Fx = a.fx[src];
+                        if constexpr (D::cy != 0) Fy = a.fy[src];
+                        if constexpr (D::cz != 0) Fz = a.fz[src];
                     }
-                    double dot = (double)D::cx * Fx + (double)D::cy * Fy;
-                    dot = dot + (double)D::cz * Fz;
+                    const double dot = cdot<q>(Fx, Fy, Fz);
                     gv = gv + 3.0 * weightE<q>() * dot;
                 }
                 ps[q] = gv;

@@ -4,6 +4,11 @@
         stream copy of a known size: the calibration kernel for FETCH_SIZE/WRITE_SIZE
     python -m droplet_amd.probe step [--grid 256] [--steps 20] [--fp32] [--shape S]
         the 256^3 shear-droplet step kernel of bench.py, a few launches
+    python -m droplet_amd.probe hit [--grid 256] [--steps 20] [--fp32] [--we 5]
+        forced isotropic turbulence with 32 droplets of radius 10 (set-B parameters,
+        Smagorinsky Cs = 0.1, spectral forcing of the shells 1 <= |k| < 3, solenoidal
+        k^-5/3 initial velocity with u_rms = 0.03, numpy.random.default_rng(3)):
+        MLUPS of the whole step (forcing + LES + step kernels)
     python -m droplet_amd.probe ensemble [--members 64] [--steps 60] [--fp64]
         64 shear-droplet members, N uniform in {48, 64, 96, 128}, radius N/5,
         Ca uniform in [0.05, 0.5], centre offsets within +-2 cells
@@ -70,9 +75,30 @@ def ensemble(a, dl):
                       "speedup": ms_loop / ms_batch}))
 
 
+def turbulence(a, dl):
+    import time
+    from droplet_amd import hit
+    from droplet_amd.abi import DRP_FP32_SHIFTED, DRP_FP64
+    N = a.grid
+    t0 = time.perf_counter()
+    phi, p, u1, u2, u3, drops = hit.initial_state(N, seed=3, u_rms=0.03, count=32, D=20.0)
+    t_init = time.perf_counter() - t0
+    cfg = hit.hit_config(N, We=a.we, D=20.0, precision=DRP_FP32_SHIFTED if a.fp32 else DRP_FP64)
+    lat = dl.Lattice(cfg)
+    lat.init_fields(phi, p, u1, u2, u3)
+    del phi, p, u1, u2, u3
+    lat.step(3)
+    ms = lat.time_steps(a.steps)
+    st = lat.reduce()
+    print(json.dumps({"workload": f"turbulent droplets {N}^3, 32 x R10, We {a.we}, LES Cs 0.1, spectral forcing kc 3",
+                      "precision": "fp32_shifted" if a.fp32 else "fp64", "steps": a.steps,
+                      "ms_per_step": ms / a.steps, "MLUPS": N ** 3 * a.steps / (ms * 1e-3) / 1e6,
+                      "kinetic": st["kinetic"], "phi_mass": st["phi_mass"], "host_init_s": round(t_init, 1)}))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "step", "autotune", "ensemble"])
+    ap.add_argument("what", choices=["copy", "step", "autotune", "ensemble", "hit"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
@@ -81,10 +107,14 @@ def main(argv=None):
     ap.add_argument("--shape", type=int, default=-1)
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--fp64", action="store_true")
+    ap.add_argument("--we", type=float, default=5.0)
     a = ap.parse_args(argv)
     from droplet_amd import lattice as dl
     if a.what == "ensemble":
         ensemble(a, dl)
+        return
+    if a.what == "hit":
+        turbulence(a, dl)
         return
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
