@@ -199,6 +199,7 @@ extern "C" int drp_batch_create(const drp_config* cfgs, int count, drp_batch** o
     drp_batch* B = new (std::nothrow) drp_batch();
     if (!B) return drp_fail(DRP_ERR_ALLOC, "host allocation failed");
     B->precision = cfgs[0].precision;
+    B->shape = default_shape(B->precision);
     B->arg_size = B->precision == DRP_FP64 ? sizeof(StepArgs) : sizeof(StepArgs32);
     int rc = DRP_OK;
     if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess)

@@ -113,14 +113,17 @@ static int validate(const drp_config* c) {
     return DRP_OK;
 }
 
-static int pick_shape(const drp_config& c) {
+// default tile shape: the fastest at 256^3 on MI355X (drp_autotune): 32x4x2 for
+// FP64 (two 512-thread workgroups per CU), 32x4x4 for FP32 (two 1024-thread
+// workgroups per CU at <= 64 VGPRs); DROPLET_SHAPE overrides
+int default_shape(int precision) {
     if (const char* e = std::getenv("DROPLET_SHAPE")) {
         int s = std::atoi(e);
         if (s >= 0 && s < SHAPE_COUNT) return s;
     }
-    (void)c;
-    return SHAPE_32x4x2;
+    return precision == DRP_FP64 ? SHAPE_32x4x2 : SHAPE_32x4x4;
 }
+static int pick_shape(const drp_config& c) { return default_shape(c.precision); }
 
 extern "C" int drp_create(const drp_config* cfg, drp_lattice** out) { return lattice_create(cfg, nullptr, out); }
 

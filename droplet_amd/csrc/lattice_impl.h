@@ -40,6 +40,7 @@ int lattice_alloc(drp_lattice* L);
 int lattice_create(const drp_config* cfg, hipStream_t stream, drp_lattice** out);
 void lattice_free(drp_lattice* L);
 drp::StepArgs make_step_args(drp_lattice* L, int64_t nstep);
+int default_shape(int precision);
 // SPECTRAL: recompute the force field from the current velocity (FP64 or FP32 fields)
 int spectral_update(drp_lattice* L, int64_t nstep);
 int reset_range(drp_lattice* L);

@@ -57,12 +57,14 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     // collision phase needs two 16-byte LDS reads and about ten FMAs per direction:
     //   fa = (At, 3 phi u1, 3 phi u2, 3 phi u3)       phi-population equilibrium
     //   ga = (3 pt - 1.5 |u|^2, u1, u2, u3)           momentum equilibrium
-    //   ka = (K0, K1, K2, -)   K_k = 4.5 g_k^2 - 1.5 |g|^2     Korteweg, axis c
-    //   pa = (9 g0 g1, 9 g0 g2, 9 g1 g2, -)                      Korteweg, diagonal c
+    //   ka = (K0, K1, K2, 9 g0 g1)   K_k = 4.5 g_k^2 - 1.5 |g|^2   Korteweg terms
+    //   pa = (9 g0 g2, 9 g1 g2)      (float2)
+    // 56 bytes per halo node: with the 32x4x2 tile, three workgroups fit one CU.
     // For a diagonal c (c_a^2 = 1) the contraction 4.5 (c.g)^2 - 1.5 |g|^2 |c|^2
     // reduces to 9 (cx cy g0 g1 + cx cz g0 g2 + cy cz g1 g2).
     __shared__ float s_phi[H2];  // phit
-    __shared__ float4 s_fa[H1], s_ga[H1], s_ka[H1], s_pa[H1];
+    __shared__ float4 s_fa[H1], s_ga[H1], s_ka[H1];
+    __shared__ float2 s_pa[H1];
     __shared__ float s_rtau[TAUMODE == 0 ? 1 : H1];  // 1 / tau_g
     __shared__ double s_red[2][NT / 64];
 
@@ -150,8 +152,8 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         s_ga[h] = make_float4(__builtin_fmaf(-1.5f, usq, 3.0f * pu.x), pu.y, pu.z, pu.w);
         const float s2h = 1.5f * gsq;
         s_ka[h] = make_float4(__builtin_fmaf(4.5f * gf0, gf0, -s2h), __builtin_fmaf(4.5f * gf1, gf1, -s2h),
-                              __builtin_fmaf(4.5f * gf2, gf2, -s2h), 0.0f);
-        s_pa[h] = make_float4(9.0f * gf0 * gf1, 9.0f * gf0 * gf2, 9.0f * gf1 * gf2, 0.0f);
+                              __builtin_fmaf(4.5f * gf2, gf2, -s2h), 9.0f * gf0 * gf1);
+        s_pa[h] = make_float2(9.0f * gf0 * gf2, 9.0f * gf1 * gf2);
         if constexpr (TAUMODE == 1) {
             float frac = (range == 0.0f) ? 0.0f : (pt - rminf) / range;
             s_rtau[h] = 1.0f / (0.5f + 3.0f * (frac * a.dnuf + a.nu1f));
@@ -166,13 +168,14 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         if constexpr (q == 0) {
             return 0.0f;
         } else if constexpr (q <= 6) {
-            const float4 k = s_ka[hu];
-            return D::cx != 0 ? k.x : (D::cy != 0 ? k.y : k.z);
+            const float* k = reinterpret_cast<const float*>(&s_ka[hu]);
+            return k[D::cx != 0 ? 0 : (D::cy != 0 ? 1 : 2)];
         } else {
-            const float4 pp = s_pa[hu];
-            float t = (D::cx * D::cy > 0) ? pp.x : -pp.x;
-            t = (D::cx * D::cz > 0) ? t + pp.y : t - pp.y;
-            return (D::cy * D::cz > 0) ? t + pp.z : t - pp.z;
+            const float p01 = reinterpret_cast<const float*>(&s_ka[hu])[3];
+            const float2 pp = s_pa[hu];
+            float t = (D::cx * D::cy > 0) ? p01 : -p01;
+            t = (D::cx * D::cz > 0) ? t + pp.x : t - pp.x;
+            return (D::cy * D::cz > 0) ? t + pp.y : t - pp.y;
         }
     };
 
@@ -303,9 +306,14 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     }
 }
 
+// Resident wavefronts per SIMD the register allocation targets: two workgroups
+// per CU for the 1024-thread tiles (8 per SIMD, <= 64 VGPRs), three for the
+// 512-thread tiles (6 per SIMD), as far as the LDS records allow.
+template <int NT> constexpr int waves_per_eu32() { return NT >= 1024 ? 8 : 6; }
+
 // one lattice: workgroup id -> tile (XCD-contiguous)
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
-__global__ __launch_bounds__(2 * TX* TY* TZ) void step_fp32(StepArgs32 a) {
+__global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu32<2 * TX * TY * TZ>()))) void step_fp32(StepArgs32 a) {
     step_tile_fp32<TX, TY, TZ, WALL, FORCE, TAUMODE>(a, xcd_tile32(blockIdx.x, gridDim.x));
 }
 
@@ -313,7 +321,7 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) void step_fp32(StepArgs32 a) {
 // tile of instance i (ascending, tile_start[0] = 0); the instance of a
 // workgroup is found by binary search, its arguments read from global memory
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
-__global__ __launch_bounds__(2 * TX* TY* TZ) void step_batch_fp32(const StepArgs32* __restrict__ args, const int* __restrict__ tile_start, int ninst) {
+__global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu32<2 * TX * TY * TZ>()))) void step_batch_fp32(const StepArgs32* __restrict__ args, const int* __restrict__ tile_start, int ninst) {
     const int b = xcd_tile32(blockIdx.x, gridDim.x);
     int lo = 0, hi = ninst - 1;
     while (lo < hi) {
