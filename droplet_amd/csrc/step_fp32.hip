@@ -57,14 +57,15 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     // collision phase needs two 16-byte LDS reads and about ten FMAs per direction:
     //   fa = (At, 3 phi u1, 3 phi u2, 3 phi u3)       phi-population equilibrium
     //   ga = (3 pt - 1.5 |u|^2, u1, u2, u3)           momentum equilibrium
-    //   ka = (K0, K1, K2, 9 g0 g1)   K_k = 4.5 g_k^2 - 1.5 |g|^2   Korteweg terms
-    //   pa = (9 g0 g2, 9 g1 g2)      (float2)
-    // 56 bytes per halo node: with the 32x4x2 tile, three workgroups fit one CU.
+    //   k[0..2] = K_k = 4.5 g_k^2 - 1.5 |g|^2                       Korteweg, axis c
+    //   k[3..5] = 9 g0 g1, 9 g0 g2, 9 g1 g2                          Korteweg, diagonal c
+    // (six float arrays, so that lanes reading consecutive nodes hit distinct
+    // banks).  56 bytes per halo node: two 1024-thread workgroups fit one CU.
     // For a diagonal c (c_a^2 = 1) the contraction 4.5 (c.g)^2 - 1.5 |g|^2 |c|^2
     // reduces to 9 (cx cy g0 g1 + cx cz g0 g2 + cy cz g1 g2).
     __shared__ float s_phi[H2];  // phit
-    __shared__ float4 s_fa[H1], s_ga[H1], s_ka[H1];
-    __shared__ float2 s_pa[H1];
+    __shared__ float4 s_fa[H1], s_ga[H1];
+    __shared__ float s_k[6][H1];
     __shared__ float s_rtau[TAUMODE == 0 ? 1 : H1];  // 1 / tau_g
     __shared__ double s_red[2][NT / 64];
 
@@ -151,9 +152,12 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         s_fa[h] = make_float4(At, ph3 * pu.y, ph3 * pu.z, ph3 * pu.w);
         s_ga[h] = make_float4(__builtin_fmaf(-1.5f, usq, 3.0f * pu.x), pu.y, pu.z, pu.w);
         const float s2h = 1.5f * gsq;
-        s_ka[h] = make_float4(__builtin_fmaf(4.5f * gf0, gf0, -s2h), __builtin_fmaf(4.5f * gf1, gf1, -s2h),
-                              __builtin_fmaf(4.5f * gf2, gf2, -s2h), 9.0f * gf0 * gf1);
-        s_pa[h] = make_float2(9.0f * gf0 * gf2, 9.0f * gf1 * gf2);
+        s_k[0][h] = __builtin_fmaf(4.5f * gf0, gf0, -s2h);
+        s_k[1][h] = __builtin_fmaf(4.5f * gf1, gf1, -s2h);
+        s_k[2][h] = __builtin_fmaf(4.5f * gf2, gf2, -s2h);
+        s_k[3][h] = 9.0f * gf0 * gf1;
+        s_k[4][h] = 9.0f * gf0 * gf2;
+        s_k[5][h] = 9.0f * gf1 * gf2;
         if constexpr (TAUMODE == 1) {
             float frac = (range == 0.0f) ? 0.0f : (pt - rminf) / range;
             s_rtau[h] = 1.0f / (0.5f + 3.0f * (frac * a.dnuf + a.nu1f));
@@ -168,14 +172,12 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         if constexpr (q == 0) {
             return 0.0f;
         } else if constexpr (q <= 6) {
-            const float* k = reinterpret_cast<const float*>(&s_ka[hu]);
-            return k[D::cx != 0 ? 0 : (D::cy != 0 ? 1 : 2)];
+            return s_k[D::cx != 0 ? 0 : (D::cy != 0 ? 1 : 2)][hu];
         } else {
-            const float p01 = reinterpret_cast<const float*>(&s_ka[hu])[3];
-            const float2 pp = s_pa[hu];
+            const float p01 = s_k[3][hu], p02 = s_k[4][hu], p12 = s_k[5][hu];
             float t = (D::cx * D::cy > 0) ? p01 : -p01;
-            t = (D::cx * D::cz > 0) ? t + pp.x : t - pp.x;
-            return (D::cy * D::cz > 0) ? t + pp.y : t - pp.y;
+            t = (D::cx * D::cz > 0) ? t + p02 : t - p02;
+            return (D::cy * D::cz > 0) ? t + p12 : t - p12;
         }
     };
 
