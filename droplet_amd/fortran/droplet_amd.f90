@@ -17,7 +17,10 @@ module droplet_amd
     integer(c_int), parameter :: DRP_OK = 0
     integer(c_int), parameter :: DRP_FP64 = 0, DRP_FP32_SHIFTED = 1
     integer(c_int), parameter :: DRP_WALL_NONE = 0, DRP_WALL_Y = 2, DRP_WALL_Z = 3
-    integer(c_int), parameter :: DRP_FORCE_NONE = 0, DRP_FORCE_TAYLOR_GREEN = 1, DRP_FORCE_FIELD = 2
+    integer(c_int), parameter :: DRP_FORCE_NONE = 0, DRP_FORCE_TAYLOR_GREEN = 1, DRP_FORCE_FIELD = 2, &
+                                 DRP_FORCE_SPECTRAL = 3
+    integer(c_int), parameter :: DRP_FILE_FIELDS = 0, DRP_FILE_FG = 1, DRP_FILE_G = 2
+    integer(c_int), parameter :: DRP_GRAD_MOBILITY = 0, DRP_GRAD_KAPPA_G = 1, DRP_GRAD_WALL_U = 2
     integer(c_int), parameter :: DRP_FIELD_PHI = 0, DRP_FIELD_P = 1, DRP_FIELD_U1 = 2, DRP_FIELD_U2 = 3, &
                                  DRP_FIELD_U3 = 4
     integer(c_int), parameter :: DRP_POP_PHI = 0, DRP_POP_MOM = 1
@@ -118,6 +121,95 @@ module droplet_amd
             integer(c_int), value :: mode
             real(c_double), intent(out) :: L, B, D
         end function
+        ! forcing arrays of fg_cal / externalforce
+        integer(c_int) function drp_set_force(lat, fx, fy, fz, n) bind(C, name="drp_set_force")
+            import :: c_int, c_ptr, c_double, c_size_t
+            type(c_ptr), value :: lat
+            real(c_double), intent(in) :: fx(*), fy(*), fz(*)
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function drp_get_force(lat, fx, fy, fz, n) bind(C, name="drp_get_force")
+            import :: c_int, c_ptr, c_double, c_size_t
+            type(c_ptr), value :: lat
+            real(c_double), intent(out) :: fx(*), fy(*), fz(*)
+            integer(c_size_t), value :: n
+        end function
+        ! output / outputfg / step_input files in the reference's per-rank layout
+        integer(c_int) function drp_write_reference_files_c(lat, dir, n, ranks_x, ranks_y, kind) &
+                bind(C, name="drp_write_reference_files")
+            import :: c_int, c_ptr, c_char, c_int64_t
+            type(c_ptr), value :: lat
+            character(kind=c_char), intent(in) :: dir(*)
+            integer(c_int64_t), value :: n
+            integer(c_int), value :: ranks_x, ranks_y, kind
+        end function
+        integer(c_int) function drp_read_reference_files_c(lat, dir, n, ranks_x, ranks_y, kind) &
+                bind(C, name="drp_read_reference_files")
+            import :: c_int, c_ptr, c_char, c_int64_t
+            type(c_ptr), value :: lat
+            character(kind=c_char), intent(in) :: dir(*)
+            integer(c_int64_t), value :: n
+            integer(c_int), value :: ranks_x, ranks_y, kind
+        end function
+        ! timing and tile shape
+        integer(c_int) function drp_time_steps(lat, nsteps, ms) bind(C, name="drp_time_steps")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lat
+            integer(c_int), value :: nsteps
+            real(c_double), intent(out) :: ms
+        end function
+        integer(c_int) function drp_set_tile_shape(lat, shape) bind(C, name="drp_set_tile_shape")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: lat
+            integer(c_int), value :: shape
+        end function
+        ! ensembles: one launch per step over all members
+        integer(c_int) function drp_batch_create(cfgs, count, batch) bind(C, name="drp_batch_create")
+            import :: c_int, c_ptr, drp_config
+            type(drp_config), intent(in) :: cfgs(*)
+            integer(c_int), value :: count
+            type(c_ptr), intent(out) :: batch
+        end function
+        type(c_ptr) function drp_batch_member(batch, i) bind(C, name="drp_batch_member")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: i
+        end function
+        integer(c_int) function drp_batch_step(batch, nsteps) bind(C, name="drp_batch_step")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function drp_batch_sync(batch) bind(C, name="drp_batch_sync")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+        end function
+        subroutine drp_batch_destroy(batch) bind(C, name="drp_batch_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: batch
+        end subroutine
+        ! parameter sensitivities (checkpointed adjoint)
+        integer(c_int) function drp_tape_create(lat, checkpoint_every, tape) bind(C, name="drp_tape_create")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: lat
+            integer(c_int), value :: checkpoint_every
+            type(c_ptr), intent(out) :: tape
+        end function
+        integer(c_int) function drp_tape_step(tape, nsteps) bind(C, name="drp_tape_step")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: tape
+            integer(c_int), value :: nsteps
+        end function
+        integer(c_int) function drp_tape_backward(tape, w, loss, grad) bind(C, name="drp_tape_backward")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: tape
+            real(c_double), intent(in) :: w(2)
+            real(c_double), intent(out) :: loss, grad(3)
+        end function
+        subroutine drp_tape_destroy(tape) bind(C, name="drp_tape_destroy")
+            import :: c_ptr
+            type(c_ptr), value :: tape
+        end subroutine
         type(c_ptr) function drp_last_error_c() bind(C, name="drp_last_error")
             import :: c_ptr
         end function
@@ -145,6 +237,23 @@ contains
     end function
 
     ! Stop with the library's message when rc is not DRP_OK.
+    ! Fortran-string wrappers of the file entry points (NUL-terminated for C)
+    integer(c_int) function drp_write_reference_files(lat, dir, n, ranks_x, ranks_y, kind)
+        type(c_ptr), intent(in) :: lat
+        character(len=*), intent(in) :: dir
+        integer(c_int64_t), intent(in) :: n
+        integer(c_int), intent(in) :: ranks_x, ranks_y, kind
+        drp_write_reference_files = drp_write_reference_files_c(lat, trim(dir)//c_null_char, n, ranks_x, ranks_y, kind)
+    end function
+
+    integer(c_int) function drp_read_reference_files(lat, dir, n, ranks_x, ranks_y, kind)
+        type(c_ptr), intent(in) :: lat
+        character(len=*), intent(in) :: dir
+        integer(c_int64_t), intent(in) :: n
+        integer(c_int), intent(in) :: ranks_x, ranks_y, kind
+        drp_read_reference_files = drp_read_reference_files_c(lat, trim(dir)//c_null_char, n, ranks_x, ranks_y, kind)
+    end function
+
     subroutine drp_check(rc, where)
         integer(c_int), intent(in) :: rc
         character(*), intent(in) :: where

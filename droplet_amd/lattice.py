@@ -56,6 +56,8 @@ def lib():
         L.drp_set_force.argtypes = [vp, dp, dp, dp, sz]
         L.drp_reduce.argtypes = [vp, C.POINTER(Stats)]
         L.drp_get_force.argtypes = [vp, dp, dp, dp, sz]
+        L.drp_write_reference_files.argtypes = [vp, C.c_char_p, C.c_int64, C.c_int, C.c_int, C.c_int]
+        L.drp_read_reference_files.argtypes = [vp, C.c_char_p, C.c_int64, C.c_int, C.c_int, C.c_int]
         L.drp_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
         L.drp_time_steps.argtypes = [vp, C.c_int, dp]
         L.drp_tile_shape_count.argtypes = []
@@ -194,6 +196,15 @@ class Lattice:
         out = [np.empty(self.shape, np.float64) for _ in range(3)]
         _check(lib().drp_get_force(self._h, *[_dp(a) for a in out], self.n), "drp_get_force")
         return tuple(out)
+
+    # -- the reference's per-rank files (output / outputfg / step_input g files)
+    def write_reference_files(self, directory: str, n: int, kind: int, ranks_x: int = 8, ranks_y: int = 1):
+        _check(lib().drp_write_reference_files(self._h, os.fsencode(directory), int(n), int(ranks_x), int(ranks_y),
+                                               int(kind)), "drp_write_reference_files")
+
+    def read_reference_files(self, directory: str, n: int, kind: int, ranks_x: int = 8, ranks_y: int = 1):
+        _check(lib().drp_read_reference_files(self._h, os.fsencode(directory), int(n), int(ranks_x), int(ranks_y),
+                                              int(kind)), "drp_read_reference_files")
 
     # -- analysis hooks
     def reduce(self) -> dict:

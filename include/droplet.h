@@ -236,6 +236,22 @@ int drp_tape_backward(drp_tape* t, const double w[2], double* loss, double grad[
 int64_t drp_tape_bytes(const drp_tape* t);
 void drp_tape_destroy(drp_tape* t);
 
+/* ---- the reference's per-rank files (Fortran unformatted sequential, 4-byte
+ * record markers; rank r holds x in [rx*nx/ranks_x, ...), y in [ry*ny/ranks_y, ...),
+ * all z, with rx = r / ranks_y, ry = r % ranks_y as program/lbm_drop_xy.f90:145-149
+ * numbers ranks; loops z, y, x (, i) in each file).
+ *   DRP_FILE_FIELDS <dir>/0_<r>_<n>.bin     records (phi, u1, u2, u3, p)    = output (:695-719)
+ *   DRP_FILE_FG     <dir>/0_<r>_<n>_fg.bin  records (f_i, g_i)               = outputfg (:721-743)
+ *   DRP_FILE_G      <dir>/0_<r>_<n>_fg.bin  records g_i (momentum only)      = step_input input (:789-828)
+ * drp_read_reference_files with DRP_FILE_FG is a restart: populations, moments,
+ * the step counter (= n) and the phi range of the restored state; with
+ * DRP_FILE_G it injects the momentum populations like drp_inject_momentum. */
+#define DRP_FILE_FIELDS 0
+#define DRP_FILE_FG 1
+#define DRP_FILE_G 2
+int drp_write_reference_files(drp_lattice* lat, const char* dir, int64_t n, int ranks_x, int ranks_y, int kind);
+int drp_read_reference_files(drp_lattice* lat, const char* dir, int64_t n, int ranks_x, int ranks_y, int kind);
+
 /* Current body-force field (DRP_FORCE_FIELD / DRP_FORCE_SPECTRAL: the field the last
  * step used) to host, three arrays of nx*ny*nz doubles.  No reference counterpart
  * (forcex/forcey/forcez are module arrays there, program/lbm_drop_iht_xy.f90:722-841). */

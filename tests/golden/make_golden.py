@@ -24,6 +24,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
 from oracle import ref_build  # noqa: E402
 
 MPIEXEC = "/opt/conda/bin/mpiexec"
@@ -34,29 +35,7 @@ E = np.array([2 / 9] + [1 / 9] * 6 + [1 / 72] * 8)
 
 
 # ------------------------------------------------------------------ Fortran I/O
-def read_records(path: str, nval: int) -> np.ndarray:
-    """Fortran unformatted sequential file of equal records of nval doubles."""
-    raw = np.fromfile(path, dtype=np.uint8)
-    rec = 8 * nval + 8
-    if raw.size % rec:
-        raise ValueError(f"{path}: size {raw.size} is not a multiple of {rec}")
-    r = raw.reshape(-1, rec)
-    head = r[:, :4].copy().view(np.int32).ravel()
-    tail = r[:, -4:].copy().view(np.int32).ravel()
-    if not ((head == 8 * nval).all() and (tail == 8 * nval).all()):
-        raise ValueError(f"{path}: bad record markers")
-    return r[:, 4:-4].copy().view(np.float64).reshape(-1, nval)
-
-
-def write_records(path: str, values: np.ndarray) -> None:
-    """One double per record (the g-only input format of program/lbm_drop_xy.f90:799)."""
-    v = np.ascontiguousarray(values, np.float64).ravel()
-    out = np.empty((v.size, 16), np.uint8)
-    marker = np.array([8], np.int32).view(np.uint8)
-    out[:, :4] = marker
-    out[:, 4:12] = v.view(np.uint8).reshape(-1, 8)
-    out[:, 12:] = marker
-    out.tofile(path)
+from fortran_io import assemble_periodic, assemble_pops, read_records, write_g_input  # noqa: E402
 
 
 def fnum(v: float) -> str:
@@ -99,29 +78,6 @@ def periodic_phys(N, eta, Re, Ca, Dv, Dr, umax):
                 force_amp=2.0 * pi * pi * umax * umax / (Dv * Re), force_k=pi / Dv)
 
 
-def assemble_periodic(outdir: str, step: int, N: int, nranks: int = 8) -> np.ndarray:
-    """output() files 0_<rank>_<n>.bin, records (phi,u1,u2,u3,p), z-y-x loops, x split over ranks."""
-    xp = N // nranks
-    out = np.zeros((5, N, N, N))
-    for r in range(nranks):
-        a = read_records(os.path.join(outdir, f"0_{r}_{step}.bin"), 5).reshape(N, N, xp, 5)
-        out[:, :, :, r * xp:(r + 1) * xp] = np.moveaxis(a, -1, 0)
-    # reorder (phi,u1,u2,u3,p) -> (phi,p,u1,u2,u3)
-    return out[[0, 4, 1, 2, 3]]
-
-
-def assemble_pops(outdir: str, step: int, N: int, nranks: int = 8) -> np.ndarray:
-    """outputfg() files, records (f_i, g_i), i fastest then x, y, z."""
-    xp = N // nranks
-    f = np.zeros((15, N, N, N))
-    g = np.zeros((15, N, N, N))
-    for r in range(nranks):
-        a = read_records(os.path.join(outdir, "fg", f"0_{r}_{step}_fg.bin"), 2).reshape(N, N, xp, 15, 2)
-        f[:, :, :, r * xp:(r + 1) * xp] = np.moveaxis(a[..., 0], -1, 0)
-        g[:, :, :, r * xp:(r + 1) * xp] = np.moveaxis(a[..., 1], -1, 0)
-    return np.stack([f, g])
-
-
 def case_periodic(name, N, steps, keep, eta, Re, Ca, Dv, Dr, umax, pops_at=None):
     params = periodic_params(N, steps, eta, Re, Ca, Dv, Dr, umax)
     exe = ref_build.build("program/lbm_drop_xy.f90", f"lbm_drop_xy_{name}", params)
@@ -129,7 +85,7 @@ def case_periodic(name, N, steps, keep, eta, Re, Ca, Dv, Dr, umax, pops_at=None)
         run(exe, 8, tmp)
         data = {f"fields_{n}": assemble_periodic(os.path.join(tmp, "out"), n, N) for n in keep}
         if pops_at is not None:
-            data[f"pops_{pops_at}"] = assemble_pops(os.path.join(tmp, "out"), pops_at, N)
+            data[f"pops_{pops_at}"] = assemble_pops(os.path.join(tmp, "out", "fg"), pops_at, N)
     meta = dict(case=name, program="program/lbm_drop_xy.f90", nranks=8, params=params,
                 phys=periodic_phys(N, eta, Re, Ca, Dv, Dr, umax), steps=list(keep),
                 sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "program/lbm_drop_xy.f90")))
@@ -153,12 +109,9 @@ def case_injection(name, N, steps, keep, step_input, seed, eta, Re, Ca, Dv, Dr, 
     for i in range(15):
         cu = CX[i] * u1 + CY[i] * u2 + CZ[i] * u3
         g[i] = E[i] * (3 * p + 3 * cu + 4.5 * cu * cu - 1.5 * (u1 * u1 + u2 * u2 + u3 * u3))
-    xp = N // 8
     with tempfile.TemporaryDirectory() as tmp:
         os.makedirs(os.path.join(tmp, "in"))
-        for r in range(8):
-            sub = g[:, :, :, r * xp:(r + 1) * xp]  # [i][z][y][x] -> records i fastest, then x, y, z
-            write_records(os.path.join(tmp, "in", f"0_{r}_7_fg.bin"), np.moveaxis(sub, 0, -1))
+        write_g_input(os.path.join(tmp, "in"), 7, g)
         run(exe, 8, tmp)
         data = {f"fields_{n}": assemble_periodic(os.path.join(tmp, "out"), n, N) for n in keep}
     data["g_injected"] = g
