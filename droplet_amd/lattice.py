@@ -65,6 +65,7 @@ def lib():
         L.drp_get_tile_shape.argtypes = [vp]
         L.drp_autotune.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_int), dp]
         L.drp_copy_bandwidth.argtypes = [C.c_size_t, C.c_int, dp]
+        L.drp_copy_bandwidth_soa.argtypes = [C.c_size_t, C.c_int, C.c_int, dp]
         L.drp_batch_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(vp)]
         L.drp_batch_size.argtypes = [vp]
         L.drp_batch_member.argtypes = [vp, C.c_int]
@@ -229,6 +230,14 @@ def autotune(cfg: Config, nsteps: int = 10):
     ms = (C.c_double * n)()
     _check(lib().drp_autotune(C.byref(cfg), int(nsteps), C.byref(best), ms), "drp_autotune")
     return best.value, list(ms)
+
+
+def copy_bandwidth_soa(bytes_per_stream: int, streams: int = 40, reps: int = 10) -> float:
+    """GB/s of `streams` SoA arrays copied at once (read + write counted)."""
+    gbs = C.c_double()
+    _check(lib().drp_copy_bandwidth_soa(int(bytes_per_stream), int(streams), int(reps), C.byref(gbs)),
+           "drp_copy_bandwidth_soa")
+    return gbs.value
 
 
 def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:

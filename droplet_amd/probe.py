@@ -2,6 +2,9 @@
 
     python -m droplet_amd.probe copy [--mib 1024] [--reps 20]
         stream copy of a known size: the calibration kernel for FETCH_SIZE/WRITE_SIZE
+    python -m droplet_amd.probe copysoa [--grid 256] [--reps 20]
+        10, 20 and 40 structure-of-arrays streams of grid^3 doubles (40: floats'
+        worth of bytes) copied at once: the achievable rate of the step's pattern
     python -m droplet_amd.probe step [--grid 256] [--steps 20] [--fp32] [--shape S]
         the 256^3 shear-droplet step kernel of bench.py, a few launches
     python -m droplet_amd.probe hit [--grid 256] [--steps 20] [--fp32] [--we 5]
@@ -98,7 +101,7 @@ def turbulence(a, dl):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "step", "autotune", "ensemble", "hit"])
+    ap.add_argument("what", choices=["copy", "copysoa", "step", "autotune", "ensemble", "hit"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
@@ -119,6 +122,13 @@ def main(argv=None):
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
         print(json.dumps({"copy_bytes": a.mib << 20, "reps": a.reps, "GB/s": round(gbs, 1)}))
+        return
+    if a.what == "copysoa":
+        n = a.grid ** 3
+        out = {"grid": a.grid}
+        for k in (10, 20, 40):
+            out[f"{k} streams GB/s"] = round(dl.copy_bandwidth_soa(8 * n if k <= 20 else 4 * n, k, a.reps), 1)
+        print(json.dumps(out))
         return
     from bench import shear_config
     cfg, drops, _ = shear_config(a.grid, precision=1 if a.fp32 else 0)

@@ -261,6 +261,11 @@ int drp_get_force(drp_lattice* lat, double* fx, double* fy, double* fz, size_t n
  * averaged over reps launches on a private stream. */
 int drp_copy_bandwidth(size_t bytes, int reps, double* gbs);
 
+/* The same for `streams` (10, 20 or 40) structure-of-arrays streams of
+ * bytes_per_stream each, copied at once with non-temporal stores: the step
+ * kernel's access pattern without its arithmetic. */
+int drp_copy_bandwidth_soa(size_t bytes_per_stream, int streams, int reps, double* gbs);
+
 /* Message for the calling thread's last failure ("" if none). */
 const char* drp_last_error(void);
 
