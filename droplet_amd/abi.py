@@ -55,7 +55,8 @@ class Config(C.Structure):
         ("force_kind", C.c_int32), ("force_start", C.c_int32),
         ("force_amp", C.c_double), ("force_k", C.c_double),
         ("les_cs", C.c_double), ("les_delta", C.c_double),
-        ("reserved", C.c_int32 * 8),
+        ("les_ranks_x", C.c_int32), ("les_ranks_y", C.c_int32), ("les_ranks_z", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
     def as_dict(self) -> dict:

@@ -88,23 +88,33 @@ __global__ void moments_fp32(int64_t n, const float* __restrict__ f, const float
     }
 }
 
+// The FP32 form of les_tau_fp64 (aux_kernels.hip): tau_g for the next step from
+// the state a step left (mac) and the one it started from (mac_old), every
+// neighbour outside the node's rank block (B, periodic wraps included) from
+// mac_old (B.bx = 0: all from mac).
 __global__ void les_tau_fp32(int nx, int ny, int nz, Phys P, const float* __restrict__ mac,
-                             const uint64_t* __restrict__ range, int slot, float* __restrict__ tau) {
+                             const float* __restrict__ mac_old, LesBlocks B, const uint64_t* __restrict__ range,
+                             int slot, float* __restrict__ tau) {
     const int64_t N = (int64_t)nx * ny * nz;
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= N) return;
     const int x = s % nx, y = (s / nx) % ny, z = s / ((int64_t)nx * ny);
+    const int x0 = B.bx > 0 ? (x / B.bx) * B.bx : 0, y0 = B.bx > 0 ? (y / B.by) * B.by : 0,
+              z0 = B.bx > 0 ? (z / B.bz) * B.bz : 0;
     float gu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     sforx<1, 15>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         using D = Dir<q>;
         int xx = x + D::cx, yy = y + D::cy, zz = z + D::cz;
+        const bool halo = B.bx > 0 && (xx < x0 || xx >= x0 + B.bx || yy < y0 || yy >= y0 + B.by || zz < z0 ||
+                                       zz >= z0 + B.bz);
+        const float* u = halo ? mac_old : mac;
         xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
         yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
         zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
         const int64_t t = xx + (int64_t)nx * (yy + (int64_t)ny * zz);
         for (int k = 0; k < 3; k++) {
-            const float v = mac[(2 + k) * N + t];
+            const float v = u[(2 + k) * N + t];
             gu[k][0] = acc<D::cx>(gu[k][0], v);
             gu[k][1] = acc<D::cy>(gu[k][1], v);
             gu[k][2] = acc<D::cz>(gu[k][2], v);
@@ -147,10 +157,11 @@ hipError_t launch_moments_fp32(int64_t n, const float* f, const float* g, float*
     hipLaunchKernelGGL(moments_fp32, grid_for(n), dim3(256), 0, st, n, f, g, mac, which);
     return hipGetLastError();
 }
-hipError_t launch_les_tau_fp32(int nx, int ny, int nz, const Phys& ph, const float* mac, const uint64_t* range,
-                               int slot, float* tau, hipStream_t st) {
+hipError_t launch_les_tau_fp32(int nx, int ny, int nz, const Phys& ph, const float* mac, const float* mac_old,
+                               const LesBlocks& blk, const uint64_t* range, int slot, float* tau, hipStream_t st) {
     const int64_t n = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(les_tau_fp32, grid_for(n), dim3(256), 0, st, nx, ny, nz, ph, mac, range, slot, tau);
+    hipLaunchKernelGGL(les_tau_fp32, grid_for(n), dim3(256), 0, st, nx, ny, nz, ph, mac, mac_old, blk, range, slot,
+                       tau);
     return hipGetLastError();
 }
 

@@ -1,7 +1,7 @@
 // aux_kernels.hip — initialisation, moment recomputation, LES relaxation-time
 // field and diagnostics reductions (FP64).  None of these is on the per-step hot
-// path except the LES tau field, which replaces the eddy-viscosity part of the
-// reference `physics` (program/LES_drop_xy.f90:661-693).
+// path except the LES tau field, the eddy-viscosity part of the reference
+// `physics` (program/LES_drop.f90:598-693), computed after each step.
 #include "d3q15.h"
 #include "kernels.h"
 
@@ -64,6 +64,16 @@ __device__ void u_stencil(const Grid& G, const double* u1, const double* u2, con
         for (int a = 0; a < 3; a++) gu[k][a] = gu[k][a] / 10.0;
 }
 
+// True when the neighbour (xx, yy, zz) (not wrapped) of node (x, y, z) lies
+// outside the node's rank block: program/LES_drop.f90 (:153-157) and
+// LES_drop_xy.f90 (:150-153) take every such neighbour from the halo exchanged
+// at the start of the step.  B.bx = 0: no decomposition, every neighbour current.
+__device__ __forceinline__ bool les_halo(const LesBlocks& B, int x, int y, int z, int xx, int yy, int zz) {
+    if (B.bx == 0) return false;
+    const int x0 = (x / B.bx) * B.bx, y0 = (y / B.by) * B.by, z0 = (z / B.bz) * B.bz;
+    return xx < x0 || xx >= x0 + B.bx || yy < y0 || yy >= y0 + B.by || zz < z0 || zz >= z0 + B.bz;
+}
+
 __global__ void init_equilibrium_fp64(Grid G, Phys P, const double* __restrict__ mac, double* __restrict__ f,
                                       double* __restrict__ g) {
     const int64_t N = (int64_t)G.nx * G.ny * G.nz;
@@ -115,14 +125,36 @@ __global__ void moments_fp64(int64_t N, const double* __restrict__ f, const doub
     }
 }
 
-__global__ void les_tau_fp64(Grid G, Phys P, const double* __restrict__ mac, const uint64_t* __restrict__ range,
-                             int slot, double* __restrict__ tau) {
+// tau_g for the next step from the state a step left (mac: phi, p, u of this
+// step; mac_old: the state the step started from) and that step's phi range:
+// grad u as grad_u_cal takes it at the end of physics (:447-477, :660), with
+// the neighbours outside a node's rank block from mac_old (les_halo), then
+// nu_e = Cs^2 Delta^2 sqrt(2 S:S) with S_ab = 0.5 (d_b u_a + d_a u_b) (strain_cal
+// :479-499, :661-677; amdflang lowers **0.5d0 to sqrt) and tau = 0.5 + 3 (nu + nu_e)
+// (:679-690).  mac_old == mac gives the tau of a state with current halos (ini_op).
+__global__ void les_tau_fp64(Grid G, Phys P, const double* __restrict__ mac, const double* __restrict__ mac_old,
+                             LesBlocks B, const uint64_t* __restrict__ range, int slot, double* __restrict__ tau) {
     const int64_t N = (int64_t)G.nx * G.ny * G.nz;
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= N) return;
     const int x = s % G.nx, y = (s / G.nx) % G.ny, z = s / ((int64_t)G.nx * G.ny);
     double gu[3][3];
-    u_stencil(G, mac + 2 * N, mac + 3 * N, mac + 4 * N, x, y, z, gu);
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) gu[k][a] = 0.0;
+    sfor<1, 15>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        using D = Dir<q>;
+        const int64_t t = G.at(x + D::cx, y + D::cy, z + D::cz);
+        const double* u = les_halo(B, x, y, z, x + D::cx, y + D::cy, z + D::cz) ? mac_old : mac;
+        const double v[3] = {u[2 * N + t], u[3 * N + t], u[4 * N + t]};
+        for (int k = 0; k < 3; k++) {
+            gu[k][0] = acc<D::cx>(gu[k][0], v[k]);
+            gu[k][1] = acc<D::cy>(gu[k][1], v[k]);
+            gu[k][2] = acc<D::cz>(gu[k][2], v[k]);
+        }
+    });
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) gu[k][a] = gu[k][a] / 10.0;
     // S_ab = 0.5 (d_b u_a + d_a u_b), nu_e = Cs^2 Delta^2 sqrt(2 S:S)  (:671-675, strain :481-501)
     double sum = 0.0;
     for (int b = 0; b < 3; b++)
@@ -208,11 +240,12 @@ hipError_t launch_moments_fp64(int nx, int ny, int nz, const double* f, const do
     return hipGetLastError();
 }
 
-hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const uint64_t* range,
-                               int slot, double* tau, hipStream_t st) {
+hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const double* mac_old,
+                               const LesBlocks& blk, const uint64_t* range, int slot, double* tau, hipStream_t st) {
     Grid G{nx, ny, nz};
     const int64_t N = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(les_tau_fp64, dim3(nblk(N, 256)), dim3(256), 0, st, G, ph, mac, range, slot, tau);
+    hipLaunchKernelGGL(les_tau_fp64, dim3(nblk(N, 256)), dim3(256), 0, st, G, ph, mac, mac_old, blk, range, slot,
+                       tau);
     return hipGetLastError();
 }
 

@@ -114,19 +114,16 @@ StepArgs32 make_step_args32(drp_lattice* L, int64_t nstep) {
 }
 
 int fp32_step(drp_lattice* L, int nsteps) {
-    const drp_config& c = L->cfg;
-    const Phys& P = L->phys;
     for (int k = 0; k < nsteps; k++) {
         const int64_t nstep = L->step + 1;
-        if (P.les)
-            HIP_TRY32(launch_les_tau_fp32(c.nx, c.ny, c.nz, P, (const float*)L->mac[L->cur], L->range,
-                                          (int)((nstep - 1) % 3), (float*)L->tau, L->stream));
         int rc = spectral_update(L, nstep);
         if (rc) return rc;
         StepArgs32 a = make_step_args32(L, nstep);
         HIP_TRY32(launch_step_fp32(a, L->shape, L->stream));
         L->cur = 1 - L->cur;
         L->step = nstep;
+        rc = les_update_tau(L, false);
+        if (rc) return rc;
     }
     return DRP_OK;
 }

@@ -93,6 +93,13 @@ struct StepArgs32 {
 };
 hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st, const BatchLaunch* batch = nullptr);
 
+// Rank blocks of the reference's LES decomposition, in nodes per axis (all 0:
+// none).  grad u for the eddy viscosity takes every neighbour outside a node's
+// block, periodic wraps included, from the state the step started from.
+struct LesBlocks {
+    int bx, by, bz;
+};
+
 // conversions between FP64 absolute and FP32 shifted representations; fref is
 // the 15 reference phi populations H_q phi1 + F_q p0(phi1) (host-computed)
 struct ShiftRef {
@@ -107,8 +114,8 @@ hipError_t launch_pops_to_shifted(int64_t n, const ShiftRef& r, const double* po
                                   hipStream_t st);
 hipError_t launch_macro_from_shifted(int64_t n, double phi1, const float* mac32, double* mac64, hipStream_t st);
 hipError_t launch_moments_fp32(int64_t n, const float* f, const float* g, float* mac, int which, hipStream_t st);
-hipError_t launch_les_tau_fp32(int nx, int ny, int nz, const Phys& ph, const float* mac, const uint64_t* range,
-                               int slot, float* tau, hipStream_t st);
+hipError_t launch_les_tau_fp32(int nx, int ny, int nz, const Phys& ph, const float* mac, const float* mac_old,
+                               const LesBlocks& blk, const uint64_t* range, int slot, float* tau, hipStream_t st);
 
 // populations := equilibria of the macroscopic fields (ini, program/lbm_drop_xy.f90:244-251)
 hipError_t launch_init_equilibrium_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* f,
@@ -116,9 +123,11 @@ hipError_t launch_init_equilibrium_fp64(int nx, int ny, int nz, const Phys& ph, 
 // recompute moments from populations: which & 1 -> phi from f, which & 2 -> p, u from g
 hipError_t launch_moments_fp64(int nx, int ny, int nz, const double* f, const double* g, double* mac, int which,
                                hipStream_t st);
-// Smagorinsky tau field (program/LES_drop_xy.f90:661-693) from u, phi and the phi range slot
-hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const uint64_t* range,
-                               int slot, double* tau, hipStream_t st);
+// Smagorinsky tau field for the next step (program/LES_drop.f90:598-693) from the
+// state a step left (mac), the one it started from (mac_old, read across rank
+// blocks) and that step's phi range slot
+hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const double* mac_old,
+                               const LesBlocks& blk, const uint64_t* range, int slot, double* tau, hipStream_t st);
 // diagnostics partial sums: NPART doubles per workgroup
 constexpr int NPART = 10;
 hipError_t launch_reduce_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* partial,

@@ -108,8 +108,17 @@ static int validate(const drp_config* c) {
     }
     if (c->anu != 0.0)
         return drp_fail(DRP_ERR_UNSUPPORTED, "anu != 0 (velocity-gradient term of geq) is not implemented");
-    for (int k = 0; k < 8; k++)
+    for (size_t k = 0; k < sizeof(c->reserved) / sizeof(c->reserved[0]); k++)
         if (c->reserved[k] != 0) return drp_fail(DRP_ERR_ARG, "reserved fields must be zero");
+    {
+        const int rx = c->les_ranks_x, ry = c->les_ranks_y, rz = c->les_ranks_z;
+        const bool off = rx == 0 && ry == 0 && rz == 0;
+        if (!off && (rx < 1 || ry < 1 || rz < 1))
+            return drp_fail(DRP_ERR_ARG, "les_ranks must be all 0 or all positive (got %d,%d,%d)", rx, ry, rz);
+        if (!off && (c->nx % rx || c->ny % ry || c->nz % rz))
+            return drp_fail(DRP_ERR_ARG, "les_ranks %d,%d,%d do not divide the grid %dx%dx%d", rx, ry, rz, c->nx,
+                            c->ny, c->nz);
+    }
     return DRP_OK;
 }
 
@@ -252,9 +261,34 @@ int reset_range(drp_lattice* L) {
     return DRP_OK;
 }
 
+// LES tau_g for the next step, in the tau field the step kernels read.  After a
+// step (fresh = false) from the state it left and the one it started from, with
+// the neighbours outside a node's rank block one step old, as the reference
+// computes it at the end of physics; for a state that replaced the lattice's
+// (initialisation, set populations, restart; fresh = true) with every neighbour
+// current, as ini_op does.  The phi range is the slot the next step reads.
+int les_update_tau(drp_lattice* L, bool fresh) {
+    if (!L->phys.les) return DRP_OK;
+    const drp_config& c = L->cfg;
+    LesBlocks B{0, 0, 0};
+    if (!fresh && c.les_ranks_x > 0) B = LesBlocks{c.nx / c.les_ranks_x, c.ny / c.les_ranks_y, c.nz / c.les_ranks_z};
+    const int slot = (int)(L->step % 3);
+    const int old = fresh ? L->cur : 1 - L->cur;
+    if (c.precision == DRP_FP64)
+        HIP_TRY(launch_les_tau_fp64(c.nx, c.ny, c.nz, L->phys, (const double*)L->mac[L->cur], (const double*)L->mac[old],
+                                    B, L->range, slot, L->tau, L->stream));
+    else
+        HIP_TRY(launch_les_tau_fp32(c.nx, c.ny, c.nz, L->phys, (const float*)L->mac[L->cur], (const float*)L->mac[old],
+                                    B, L->range, slot, (float*)L->tau, L->stream));
+    return DRP_OK;
+}
+
 static int init_from_host_fields(drp_lattice* L, const double* const* host) {
     const int64_t n = L->n;
-    if (L->cfg.precision != DRP_FP64) return fp32_init_from_host_fields(L, host);
+    if (L->cfg.precision != DRP_FP64) {
+        const int rc = fp32_init_from_host_fields(L, host);
+        return rc ? rc : les_update_tau(L, true);
+    }
     for (int k = 0; k < 5; k++)
         HIP_TRY(hipMemcpyAsync((double*)L->mac[L->cur] + k * n, host[k], n * sizeof(double), hipMemcpyHostToDevice,
                                L->stream));
@@ -267,7 +301,7 @@ static int init_from_host_fields(drp_lattice* L, const double* const* host) {
     L->step = 0;
     L->initialized = true;
     L->eq_init = true;
-    return DRP_OK;
+    return les_update_tau(L, true);
 }
 
 extern "C" int drp_init_droplets(drp_lattice* L, const drp_drop* drops, int ndrops, double p_init) {
@@ -363,15 +397,14 @@ extern "C" int drp_step(drp_lattice* L, int nsteps) {
     if (L->cfg.precision != DRP_FP64) return fp32_step(L, nsteps);
     for (int k = 0; k < nsteps; k++) {
         const int64_t nstep = L->step + 1;
-        if (L->phys.les)
-            HIP_TRY(launch_les_tau_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, L->phys, (const double*)L->mac[L->cur],
-                                        L->range, (int)((nstep - 1) % 3), L->tau, L->stream));
         int rc = spectral_update(L, nstep);
         if (rc) return rc;
         StepArgs a = make_step_args(L, nstep);
         HIP_TRY(launch_step_fp64(a, L->shape, L->stream));
         L->cur = 1 - L->cur;
         L->step = nstep;
+        rc = les_update_tau(L, false);
+        if (rc) return rc;
     }
     return DRP_OK;
 }
@@ -429,12 +462,18 @@ extern "C" int drp_get_populations(drp_lattice* L, int set, double* host, size_t
     return DRP_OK;
 }
 
-static int set_pops(drp_lattice* L, int set, const double* host, size_t n, int which) {
+// Populations from the host, moments recomputed (which: 1 phi, 2 p and u).  A
+// replaced state gets a fresh LES tau; an injection (program/LES_drop.f90:919-945)
+// keeps the tau the last step left.
+static int set_pops(drp_lattice* L, int set, const double* host, size_t n, int which, bool replace) {
     if (!L || !host) return drp_fail(DRP_ERR_ARG, "null argument");
     if (set != DRP_POP_PHI && set != DRP_POP_MOM) return drp_fail(DRP_ERR_ARG, "unknown population set %d", set);
     if (n != 15 * (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != 15 x %lld", n, (long long)L->n);
     if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
-    if (L->cfg.precision != DRP_FP64) return fp32_set_populations(L, set, host, which);
+    if (L->cfg.precision != DRP_FP64) {
+        const int rc = fp32_set_populations(L, set, host, which);
+        return rc || !replace ? rc : les_update_tau(L, true);
+    }
     double* base = (double*)L->pop[L->cur];
     double* dst = base + (set == DRP_POP_MOM ? 15 * L->n : 0);
     L->eq_init = false;
@@ -442,15 +481,15 @@ static int set_pops(drp_lattice* L, int set, const double* host, size_t n, int w
     HIP_TRY(launch_moments_fp64(L->cfg.nx, L->cfg.ny, L->cfg.nz, base, base + 15 * L->n, (double*)L->mac[L->cur],
                                 which, L->stream));
     HIP_TRY(hipStreamSynchronize(L->stream));
-    return DRP_OK;
+    return replace ? les_update_tau(L, true) : DRP_OK;
 }
 
 extern "C" int drp_set_populations(drp_lattice* L, int set, const double* host, size_t n) {
-    return set_pops(L, set, host, n, set == DRP_POP_PHI ? 1 : 2);
+    return set_pops(L, set, host, n, set == DRP_POP_PHI ? 1 : 2, true);
 }
 
 extern "C" int drp_inject_momentum(drp_lattice* L, const double* g_host, size_t n) {
-    return set_pops(L, DRP_POP_MOM, g_host, n, 2);
+    return set_pops(L, DRP_POP_MOM, g_host, n, 2, false);
 }
 
 extern "C" int drp_set_force(drp_lattice* L, const double* fx, const double* fy, const double* fz, size_t n) {

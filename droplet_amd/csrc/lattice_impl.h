@@ -44,6 +44,9 @@ int default_shape(int precision);
 // SPECTRAL: recompute the force field from the current velocity (FP64 or FP32 fields)
 int spectral_update(drp_lattice* L, int64_t nstep);
 int reset_range(drp_lattice* L);
+// LES tau for the next step: after a step (fresh = false, rank-block halos one
+// step old) or for a replaced state (fresh = true, all neighbours current)
+int les_update_tau(drp_lattice* L, bool fresh);
 
 // deformation.cpp
 int deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do);

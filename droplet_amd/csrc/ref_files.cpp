@@ -169,7 +169,7 @@ extern "C" int drp_read_reference_files(drp_lattice* L, const char* dir, int64_t
             drp::launch_range_set(L->range, (s0 + 2) % 3, c.phi1, c.phi2, L->stream) != hipSuccess ||
             hipStreamSynchronize(L->stream) != hipSuccess)
             return drp_fail(DRP_ERR_HIP, "phi range reset failed");
-        return DRP_OK;
+        return les_update_tau(L, true);
     }
     return drp_inject_momentum(L, g.data(), 15 * N);
 }

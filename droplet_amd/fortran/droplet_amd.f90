@@ -32,7 +32,8 @@ module droplet_amd
         real(c_double) :: wall_u
         integer(c_int32_t) :: force_kind, force_start
         real(c_double) :: force_amp, force_k, les_cs, les_delta
-        integer(c_int32_t) :: reserved(8)
+        integer(c_int32_t) :: les_ranks_x, les_ranks_y, les_ranks_z
+        integer(c_int32_t) :: reserved(5)
     end type drp_config
 
     type, bind(C) :: drp_drop

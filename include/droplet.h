@@ -90,7 +90,16 @@ typedef struct drp_config {
     double force_k;       /* Taylor-Green wavenumber pi/D_vortex (ref kw); spectral: kc (2..5) */
     double les_cs;        /* Smagorinsky constant Cs (0 = no LES)                      */
     double les_delta;     /* filter width Delta                                        */
-    int32_t reserved[8];  /* must be zero                                              */
+    /* LES: the reference computes grad u for the eddy viscosity at the end of each
+       step on its MPI rank blocks, taking every neighbour outside a node's block
+       (periodic wraps included) from the halo exchanged at the step's start, i.e.
+       one step old.  ranks = 0,0,0: all neighbours current; otherwise each count
+       >= 1 and divides its axis.  program/LES_drop.f90 runs 8 x 1 x Nz blocks,
+       program/LES_drop_xy.f90 4 x Ny x 1. */
+    int32_t les_ranks_x;
+    int32_t les_ranks_y;
+    int32_t les_ranks_z;
+    int32_t reserved[5];  /* must be zero                                              */
 } drp_config;
 
 typedef struct drp_drop {

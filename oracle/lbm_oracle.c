@@ -64,6 +64,7 @@ typedef struct ora_lattice {
     double *lap, *gp;     /* lap phi, grad phi [3][n] */
     double *gu;           /* grad u [9][n], gu[k*3+a] = d_a u_k (ref grad_u(k,a)) */
     double *tau;          /* tau_g */
+    double *uo1, *uo2, *uo3; /* u at the start of the current step (LES stale halos) */
     double phi_min, phi_max; /* range used by nu(phi) in the next step */
     int threads;
 } ora_lattice;
@@ -81,6 +82,10 @@ static inline int64_t upstream(const ora_lattice* L, int x, int y, int z, int i)
 
 ora_lattice* ora_create(const drp_config* c) {
     if (!c || c->nx < 3 || c->ny < 3 || c->nz < 3) return NULL;
+    if (c->les_ranks_x || c->les_ranks_y || c->les_ranks_z) {  /* rank blocks: all >= 1, dividing the grid */
+        if (c->les_ranks_x < 1 || c->les_ranks_y < 1 || c->les_ranks_z < 1) return NULL;
+        if (c->nx % c->les_ranks_x || c->ny % c->les_ranks_y || c->nz % c->les_ranks_z) return NULL;
+    }
     ora_lattice* L = (ora_lattice*)calloc(1, sizeof(ora_lattice));
     if (!L) return NULL;
     L->c = *c;
@@ -88,8 +93,9 @@ ora_lattice* ora_create(const drp_config* c) {
     int64_t n = L->n;
     double** bufs[] = {&L->f, &L->g, &L->fn, &L->gn, &L->feq, &L->geq};
     for (int k = 0; k < 6; k++) *bufs[k] = (double*)calloc((size_t)(15 * n), sizeof(double));
-    double** flds[] = {&L->phi, &L->p, &L->u1, &L->u2, &L->u3, &L->fx, &L->fy, &L->fz, &L->lap, &L->tau};
-    for (int k = 0; k < 10; k++) *flds[k] = (double*)calloc((size_t)n, sizeof(double));
+    double** flds[] = {&L->phi, &L->p, &L->u1, &L->u2, &L->u3, &L->fx, &L->fy, &L->fz, &L->lap, &L->tau,
+                       &L->uo1, &L->uo2, &L->uo3};
+    for (int k = 0; k < 13; k++) *flds[k] = (double*)calloc((size_t)n, sizeof(double));
     L->gp = (double*)calloc((size_t)(3 * n), sizeof(double));
     L->gu = (double*)calloc((size_t)(9 * n), sizeof(double));
     L->phi_min = c->phi1;
@@ -101,7 +107,7 @@ ora_lattice* ora_create(const drp_config* c) {
 void ora_destroy(ora_lattice* L) {
     if (!L) return;
     double* all[] = {L->f, L->g, L->fn, L->gn, L->feq, L->geq, L->phi, L->p, L->u1, L->u2, L->u3,
-                     L->fx, L->fy, L->fz, L->lap, L->tau, L->gp, L->gu};
+                     L->fx, L->fy, L->fz, L->lap, L->tau, L->gp, L->gu, L->uo1, L->uo2, L->uo3};
     for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); k++) free(all[k]);
     free(L);
 }
@@ -136,17 +142,58 @@ static void stencils_at(ora_lattice* L, int x, int y, int z) {
     }
 }
 
-/* Smagorinsky eddy viscosity from grad u: program/LES_drop_xy.f90:661-675 (strain :481-501) */
-static double eddy_viscosity(const ora_lattice* L, int64_t s) {
-    if (L->c.les_cs == 0.0 && L->c.les_delta == 0.0) return 0.0;
+static int les_on(const ora_lattice* L) { return L->c.les_cs != 0.0 && L->c.les_delta != 0.0; }
+
+/* Smagorinsky eddy viscosity from gu[k][a] = d_a u_k: program/LES_drop.f90:661-675
+   (strain :479-499; loops beta outer, alpha inner) */
+static double eddy_viscosity_of(const ora_lattice* L, double gu[3][3]) {
     double acc = 0.0;
     for (int b = 0; b < 3; b++)
         for (int a = 0; a < 3; a++) {
-            double sab = 0.5 * (L->gu[(a * 3 + b) * L->n + s] + L->gu[(b * 3 + a) * L->n + s]);
+            double sab = 0.5 * (gu[a][b] + gu[b][a]);
             acc = acc + sab * sab;
         }
     double nue = sqrt(2.0 * acc);
     return L->c.les_cs * L->c.les_cs * L->c.les_delta * L->c.les_delta * nue;
+}
+
+static double eddy_viscosity(const ora_lattice* L, int64_t s) {
+    if (!les_on(L)) return 0.0;
+    double gu[3][3];
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) gu[k][a] = L->gu[(k * 3 + a) * L->n + s];
+    return eddy_viscosity_of(L, gu);
+}
+
+/* grad u for the eddy viscosity as physics takes it at the end of a step
+   (grad_u_cal, program/LES_drop.f90:447-477, :660; LES_drop_xy.f90 grad_cal):
+   interior values are this step's u, halo values the ones exchanged at the start
+   of the step.  With the reference's decomposition into les_ranks_x x les_ranks_y
+   x les_ranks_z blocks every neighbour outside a node's block (a rank boundary or
+   a periodic wrap) therefore reads the previous step's u; with les_ranks all 0
+   every neighbour is current. */
+static void grad_u_end_of_step(const ora_lattice* L, int x, int y, int z, double gu[3][3]) {
+    const drp_config* c = &L->c;
+    const int stale_on = c->les_ranks_x > 0;
+    const int xp = stale_on ? c->nx / c->les_ranks_x : c->nx, yp = stale_on ? c->ny / c->les_ranks_y : c->ny,
+              zp = stale_on ? c->nz / c->les_ranks_z : c->nz;
+    const int bx0 = (x / xp) * xp, by0 = (y / yp) * yp, bz0 = (z / zp) * zp;
+    for (int k = 0; k < 3; k++)
+        for (int a = 0; a < 3; a++) gu[k][a] = 0.0;
+    for (int a = 0; a < 3; a++) {
+        const int* ca = a == 0 ? CX : (a == 1 ? CY : CZ);
+        for (int i = 1; i < 15; i++) {
+            const int xx = x + CX[i], yy = y + CY[i], zz = z + CZ[i];
+            const int stale = stale_on && (xx < bx0 || xx >= bx0 + xp || yy < by0 || yy >= by0 + yp || zz < bz0 ||
+                                           zz >= bz0 + zp);
+            const int64_t t = nbr(L, x, y, z, i);
+            const double cr = (double)ca[i];
+            gu[0][a] = gu[0][a] + cr * (stale ? L->uo1[t] : L->u1[t]);
+            gu[1][a] = gu[1][a] + cr * (stale ? L->uo2[t] : L->u2[t]);
+            gu[2][a] = gu[2][a] + cr * (stale ? L->uo3[t] : L->u3[t]);
+        }
+        for (int k = 0; k < 3; k++) gu[k][a] = gu[k][a] / 10.0;
+    }
 }
 
 /* nu(phi), tau_g: physics, program/lbm_drop_xy.f90:576-587 (LES form LES_drop_xy.f90:682-693) */
@@ -209,7 +256,7 @@ static void equilibria_at(ora_lattice* L, int64_t s) {
     }
 }
 
-static void compute_equilibria(ora_lattice* L) {
+static void compute_equilibria(ora_lattice* L, int with_tau) {
     const int nx = L->c.nx, ny = L->c.ny, nz = L->c.nz;
 #pragma omp parallel for num_threads(L->threads) schedule(static)
     for (int z = 0; z < nz; z++)
@@ -218,8 +265,27 @@ static void compute_equilibria(ora_lattice* L) {
 #pragma omp parallel for num_threads(L->threads) schedule(static)
     for (int64_t s = 0; s < L->n; s++) {
         equilibria_at(L, s);
-        L->tau[s] = relaxation_time(L, s);
+        if (with_tau) L->tau[s] = relaxation_time(L, s);
     }
+}
+
+/* LES: tau for the next step at the end of this one (physics, program/LES_drop.f90:598-693),
+   from the new phi, its range and grad u as grad_u_end_of_step takes it */
+static void les_tau_update(ora_lattice* L) {
+    const int nx = L->c.nx, ny = L->c.ny, nz = L->c.nz;
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                const int64_t s = idx(L, x, y, z);
+                double gu[3][3];
+                grad_u_end_of_step(L, x, y, z, gu);
+                double range = L->phi_max - L->phi_min;
+                double frac = (L->phi[s] - L->phi_min) / range;
+                if (range == 0.0) frac = 0.0;
+                double nu = frac * (L->c.nu2 - L->c.nu1) + L->c.nu1;
+                L->tau[s] = 0.5 + 3.0 * (nu + eddy_viscosity_of(L, gu)) + 2.0 / 3.0 * L->c.anu;
+            }
 }
 
 /* moments phi = sum f, p = sum g/3, u = sum c g: physics, program/lbm_drop_xy.f90:541-553 */
@@ -337,7 +403,7 @@ static void init_equilibrium_populations(ora_lattice* L) {
     /* ini: f := feq, g := geq (program/lbm_drop_xy.f90:244-251) */
     L->phi_min = L->c.phi1; /* ini_op uses (phi1, phi2) as the range, :273-284 */
     L->phi_max = L->c.phi2;
-    compute_equilibria(L);
+    compute_equilibria(L, 1);   /* ini_op: tau of the initial fields, fresh halos */
     memcpy(L->f, L->feq, sizeof(double) * 15 * L->n);
     memcpy(L->g, L->geq, sizeof(double) * 15 * L->n);
     L->step = 0;
@@ -440,15 +506,21 @@ int ora_step(ora_lattice* L, int nsteps) {
             ora_spectral_force_fields(L, L->u1, L->u2, L->u3, L->fx, L->fy, L->fz);
             force_on = 1;
         }
-        compute_equilibria(L);      /* various_cal, gphi_cal, equilibrium_cal (+ tau of physics) */
+        /* various_cal, gphi_cal, equilibrium_cal (+ tau of physics; with LES tau comes
+           from the end of the previous step) */
+        compute_equilibria(L, !les_on(L));
         stream_collide(L, force_on); /* fg_cal */
         bounce_back(L);              /* bounce_back_LBM */
         double* t;
         t = L->f; L->f = L->fn; L->fn = t;
         t = L->g; L->g = L->gn; L->gn = t;
+        memcpy(L->uo1, L->u1, sizeof(double) * (size_t)L->n);
+        memcpy(L->uo2, L->u2, sizeof(double) * (size_t)L->n);
+        memcpy(L->uo3, L->u3, sizeof(double) * (size_t)L->n);
 #pragma omp parallel for num_threads(L->threads) schedule(static)
         for (int64_t s = 0; s < L->n; s++) moments_at(L, s, 3);
         phi_range(L);
+        if (les_on(L)) les_tau_update(L);
         L->step = nstep;
     }
     return 0;
@@ -479,9 +551,17 @@ int ora_get_populations(ora_lattice* L, int set, double* out) {
     return 0;
 }
 
+/* A replaced state: moments recomputed; with LES, tau_g of the new state with
+   every neighbour current (as ini_op computes it) and the last phi range. */
 int ora_set_populations(ora_lattice* L, int set, const double* in) {
     memcpy(set == DRP_POP_PHI ? L->f : L->g, in, sizeof(double) * 15 * (size_t)L->n);
     for (int64_t s = 0; s < L->n; s++) moments_at(L, s, set == DRP_POP_PHI ? 1 : 2);
+    if (les_on(L)) {
+        memcpy(L->uo1, L->u1, sizeof(double) * (size_t)L->n);
+        memcpy(L->uo2, L->u2, sizeof(double) * (size_t)L->n);
+        memcpy(L->uo3, L->u3, sizeof(double) * (size_t)L->n);
+        les_tau_update(L);
+    }
     return 0;
 }
 
@@ -745,9 +825,15 @@ int ora_stencils(ora_lattice* L, double* lap, double* gphi, double* gu) {
 
 /* Equilibria of the current fields and range (equilibrium_cal :421-476) and tau_g. */
 int ora_equilibria(ora_lattice* L, double* feq, double* geq, double* tau) {
-    compute_equilibria(L);
+    /* the relaxation time the next step will use (with LES, from the end of the
+       last step) stays untouched */
+    double* kept = (double*)malloc(sizeof(double) * (size_t)L->n);
+    memcpy(kept, L->tau, sizeof(double) * (size_t)L->n);
+    compute_equilibria(L, 1);
     memcpy(feq, L->feq, sizeof(double) * 15 * (size_t)L->n);
     memcpy(geq, L->geq, sizeof(double) * 15 * (size_t)L->n);
     memcpy(tau, L->tau, sizeof(double) * (size_t)L->n);
+    memcpy(L->tau, kept, sizeof(double) * (size_t)L->n);
+    free(kept);
     return 0;
 }

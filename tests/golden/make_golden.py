@@ -35,7 +35,8 @@ E = np.array([2 / 9] + [1 / 9] * 6 + [1 / 72] * 8)
 
 
 # ------------------------------------------------------------------ Fortran I/O
-from fortran_io import assemble_periodic, assemble_pops, read_records, write_g_input  # noqa: E402
+from fortran_io import (assemble_periodic, assemble_pops, assemble_xz, read_records, write_fg_input,  # noqa: E402
+                        write_g_input)
 
 
 def fnum(v: float) -> str:
@@ -147,6 +148,55 @@ def case_walls(name, N, steps, D, nu, uw, kappag):
     return data, meta
 
 
+def injected_momentum(N, seed, amp=0.02, noise=0.002):
+    """A momentum-population field to inject: equilibria of a Taylor-Green-like
+    velocity plus seeded noise (numpy.random.default_rng(seed))."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    k = 2 * np.pi / N
+    u1 = amp * np.sin(k * x) * np.cos(k * z) + noise * rng.standard_normal((N, N, N))
+    u2 = noise * rng.standard_normal((N, N, N))
+    u3 = -amp * np.cos(k * x) * np.sin(k * z) + noise * rng.standard_normal((N, N, N))
+    p = 1.0 / 3.0 + 1e-3 * rng.standard_normal((N, N, N))
+    g = np.empty((15, N, N, N))
+    for i in range(15):
+        cu = CX[i] * u1 + CY[i] * u2 + CZ[i] * u3
+        g[i] = E[i] * (3 * p + 3 * cu + 4.5 * cu * cu - 1.5 * (u1 * u1 + u2 * u2 + u3 * u3))
+    return g
+
+
+def case_les(name, N, steps, step_input, seed, delta, Cs, Re, Dv, D, sigma, umax=0.1):
+    """Smagorinsky LES with a droplet, program/LES_drop.f90 on 16 ranks (8 slabs in x
+    times 2 in z; its z-neighbour set-up needs at least 2 z slabs, :184-192): momentum populations injected at step_input from (f, g) pair files (the
+    f values are ignored, :921-945), Taylor-Green force from then on (:774-798),
+    eddy viscosity from grad u at the end of each step with halos exchanged at its
+    start (physics :598-693).  Fields written when n is a multiple of 500."""
+    assert steps % 500 == 0, "LES_drop.f90 writes fields only when mod(n,500) == 0"
+    params = dict(xmax=str(N - 1), ymax=str(N - 1), zmax=str(N - 1), step=str(steps), step_input=str(step_input),
+                  step_input_file_num="7", datadir_input='"./in/"', datadir='"./out/"', datadir2='"./out/fg/"',
+                  delta=fnum(delta) + "*ds", Cs=fnum(Cs), Re=fnum(Re), D_vortex=fnum(Dv), D=fnum(D),
+                  sigma=fnum(sigma), umax=fnum(umax))
+    exe = ref_build.build("program/LES_drop.f90", f"les_drop_{name}", params)
+    g = injected_momentum(N, seed)
+    f_dummy = np.full_like(g, 12345.0)
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "in"))
+        write_fg_input(os.path.join(tmp, "in"), 7, f_dummy, g, 8, 2)
+        run(exe, 16, tmp)
+        fields = assemble_xz(os.path.join(tmp, "out"), steps, N, 8, 2)
+    pi = math.acos(-1.0)
+    nu1 = umax * Dv / Re
+    data = {f"fields_{steps}": fields, "g_injected": g}
+    meta = dict(case=name, program="program/LES_drop.f90", nranks=16, params=params, step_input=step_input,
+                phys=dict(nx=N, ny=N, nz=N, nu1=nu1, nu2=1.0 * nu1, tau_f=0.7, kappa_f=0.01,
+                          kappa_g=(sigma / 1.7039) ** (1.0 / 0.9991), drop=[0.5 * (N - 1)] * 3 + [D], p_init=1.0 / 3.0,
+                          force_amp=2.0 * pi * pi * umax * umax / (Dv * Re), force_k=pi / Dv,
+                          les_cs=Cs, les_delta=delta * 1.0, les_ranks_x=8, les_ranks_y=1, les_ranks_z=2),
+                steps=[steps],
+                sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "program/LES_drop.f90")))
+    return data, meta
+
+
 def save(name, data, meta):
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, meta=np.array(json.dumps(meta)), **data)
@@ -167,6 +217,9 @@ CASES = {
     # into the bulk near step 9450, after which the unbounded axis walk of the program's
     # deformation measure (:905-944) never exits
     "walls24": lambda: case_walls("walls24", 24, 30000, D=12.0, nu=0.1, uw=0.01, kappag=0.01),
+    # Smagorinsky LES (Cs 0.173, Delta 4) with a droplet, injection + Taylor-Green force, 500 steps
+    "les16": lambda: case_les("les16", 16, 500, step_input=3, seed=21, delta=4.0, Cs=0.173, Re=80.0, Dv=8.0,
+                              D=8.0, sigma=1e-4),
 }
 
 

@@ -27,9 +27,12 @@ def config_for(meta: dict, **over):
     """drp_config reproducing the reference parameter set of a golden case."""
     ph = meta["phys"]
     kw = dict(nu1=ph["nu1"], nu2=ph["nu2"], kappa_g=ph["kappa_g"], tau_f=ph["tau_f"], kappa_f=ph["kappa_f"])
-    if meta["case"].startswith("inject"):
+    if meta["case"].startswith(("inject", "les")):
         kw.update(force_kind=DRP_FORCE_TAYLOR_GREEN, force_start=meta["step_input"],
                   force_amp=ph["force_amp"], force_k=ph["force_k"])
+    if "les_cs" in ph:
+        kw.update(les_cs=ph["les_cs"], les_delta=ph["les_delta"], les_ranks_x=ph["les_ranks_x"],
+                  les_ranks_y=ph["les_ranks_y"], les_ranks_z=ph["les_ranks_z"])
     if "wall_u" in ph:
         kw.update(wall_axis=DRP_WALL_Z, wall_u=ph["wall_u"], wall_start=ph["wall_start"])
     kw.update(over)

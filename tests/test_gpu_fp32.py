@@ -96,6 +96,22 @@ def test_fp32_golden_injection_force():
     assert np.abs(got["u1"] - ref[2]).max() < 1e-6
 
 
+def test_fp32_les_golden():
+    """The LES fixture (program/LES_drop.f90, 8 x 1 x 2 rank blocks) in FP32."""
+    z, meta = gd.load("les16")
+    lat = Lattice(gd.config_for(meta, precision=DRP_FP32_SHIFTED))
+    lat.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    lat.step(meta["step_input"] - 1)
+    lat.inject_momentum(z["g_injected"])
+    lat.step(500 - lat.step_count)
+    ref = z["fields_500"]
+    got = lat.fields()
+    e_phi = np.abs(got["phi"] - ref[0]).max() / 4.895
+    e_u = max(np.abs(got[k] - ref[i]).max() for i, k in ((2, "u1"), (3, "u2"), (4, "u3")))
+    print(f"FP32 LES golden 16^3 @500: phi rel Linf {e_phi:.3e}, u Linf {e_u:.3e}")
+    assert e_phi < 1e-4 and e_u < 1e-6
+
+
 def test_fp32_populations_roundtrip():
     N = 16
     cfg = make_config(N, precision=DRP_FP32_SHIFTED, kappa_g=0.01)

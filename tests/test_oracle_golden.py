@@ -68,3 +68,34 @@ def test_moving_walls_bit_exact():
     got = o.fields()
     np.testing.assert_array_equal(got["phi"], z[f"phi_{n}"])
     np.testing.assert_array_equal(got["p"], z[f"p_{n}"])
+
+
+def test_les_injection_bit_exact():
+    """program/LES_drop.f90: Smagorinsky eddy viscosity from grad u at the end of each
+    step, with the values across its 8 x 2 slab boundaries and periodic wraps one
+    step old (les_ranks 8 x 1 x 2), after a momentum injection and with
+    the Taylor-Green force; 500 steps at 16^3."""
+    z, meta = gd.load("les16")
+    o = OracleLattice(gd.config_for(meta))
+    o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    o.step(meta["step_input"] - 1)
+    o.inject_momentum(z["g_injected"])
+    for n in meta["steps"]:
+        _run_to(o, n)
+        ref = z[f"fields_{n}"]
+        got = o.fields()
+        for k, fld in enumerate(FIELDS):
+            np.testing.assert_array_equal(got[fld], ref[k], err_msg=f"les16 step {n} {fld}")
+
+
+def test_les_fixture_pins_the_decomposition():
+    """The same run with every neighbour current, or with the z split left out,
+    misses the fixture: it pins the one-step-old values across slab boundaries."""
+    z, meta = gd.load("les16")
+    for over in (dict(les_ranks_x=0, les_ranks_y=0, les_ranks_z=0), dict(les_ranks_x=8, les_ranks_y=1, les_ranks_z=1)):
+        o = OracleLattice(gd.config_for(meta, **over))
+        o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+        o.step(meta["step_input"] - 1)
+        o.inject_momentum(z["g_injected"])
+        o.step(500 - o.step_count)
+        assert np.abs(o.fields()["phi"] - z["fields_500"][0]).max() > 1e-5, over
