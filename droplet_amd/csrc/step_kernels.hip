@@ -73,6 +73,9 @@ template <bool WALL, int FORCE> constexpr int waves_per_eu() { return (WALL && F
 // 15 momentum populations.  Halving the per-thread state doubles the resident
 // wavefronts per CU, and the shared staging / stencil phases are spread over
 // twice the threads.
+// workgroups of `bytes` of LDS that fit in one CU (160 KiB, 1280-byte granules)
+constexpr int wg_per_cu_lds(int bytes) { return (160 * 1024) / (((bytes + 1279) / 1280) * 1280); }
+
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
 __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     constexpr int NN = TX * TY * TZ, NT = 2 * NN;
@@ -83,12 +86,14 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     __shared__ double s_phi[H2];
     __shared__ double s_p[H1], s_u1[H1], s_u2[H1], s_u3[H1];
     __shared__ double s_A[H1], s_g0[H1], s_g1[H1], s_g2[H1];
-    // per-node 1.5 |grad phi|^2 and |u|^2, formed once with the same operations
-    // the per-direction equilibria would repeat (bit-identical), where the LDS
-    // budget of the tile shape allows
+    // per-node 1.5 |grad phi|^2 (PRE_S) and |u|^2 (PRE_U), formed once with the
+    // same operations the per-direction equilibria would repeat (bit-identical),
+    // each only where it does not lower the number of workgroups that fit in a
+    // CU's 160 KiB of LDS (sizes rounded up to a 1280-byte allocation granule)
     constexpr int LDS_BASE = 8 * (H2 + 8 * H1 + (TAUMODE == TAU_CONST ? 1 : H1) + 2 * (NT / 64));
-    constexpr bool PRE = LDS_BASE + 16 * H1 <= 160 * 1024;
-    __shared__ double s_s2h[PRE ? H1 : 1], s_usq[PRE ? H1 : 1];
+    constexpr bool PRE_S = wg_per_cu_lds(LDS_BASE + 8 * H1) >= wg_per_cu_lds(LDS_BASE);
+    constexpr bool PRE_U = PRE_S && wg_per_cu_lds(LDS_BASE + 16 * H1) >= wg_per_cu_lds(LDS_BASE);
+    __shared__ double s_s2h[PRE_S ? H1 : 1], s_usq[PRE_U ? H1 : 1];
     __shared__ double s_tau[TAUMODE == TAU_CONST ? 1 : H1];
     __shared__ double s_red[2][NT / 64];
 
@@ -175,10 +180,8 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
         const double p0 = ph * a.T / (1.0 - a.b * ph) - a.a * (ph * ph);
         s_A[h] = p0 - a.kf * ph * lap - a.kf6 * (g0 * g0 + g1 * g1 + g2 * g2);
         s_g0[h] = g0;
-        if constexpr (PRE) {
-            s_s2h[h] = 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
-            s_usq[h] = s_u1[h] * s_u1[h] + s_u2[h] * s_u2[h] + s_u3[h] * s_u3[h];
-        }
+        if constexpr (PRE_S) s_s2h[h] = 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
+        if constexpr (PRE_U) s_usq[h] = s_u1[h] * s_u1[h] + s_u2[h] * s_u2[h] + s_u3[h] * s_u3[h];
         s_g1[h] = g1;
         s_g2[h] = g2;
         if constexpr (TAUMODE == TAU_PHI) {
@@ -212,7 +215,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                     nd.g0 = s_g0[hu];
                     nd.g1 = s_g1[hu];
                     nd.g2 = s_g2[hu];
-                    const double s2h = PRE ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
+                    const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
                     gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
                 }
                 const double fe = feq<q>(nd, gt, a.kf);
@@ -234,13 +237,13 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 nd.u1 = s_u1[hu];
                 nd.u2 = s_u2[hu];
                 nd.u3 = s_u3[hu];
-                const double usq = PRE ? s_usq[hu] : nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
+                const double usq = PRE_U ? s_usq[hu] : nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
                 double gt = 0.0;
                 if constexpr (q != 0) {
                     nd.g0 = s_g0[hu];
                     nd.g1 = s_g1[hu];
                     nd.g2 = s_g2[hu];
-                    const double s2h = PRE ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
+                    const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
                     gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
                 }
                 const double ge = geq<q>(nd, usq, gt, a.kg);
