@@ -99,9 +99,12 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t N = a.n;
+    // tiles run x fastest, then z, then y: the workgroups resident at one time
+    // on an XCD (a contiguous range of tiles) cover consecutive z-layers of one
+    // tile row, so the z halos (the deeper ones: TZ <= TY) are read from L2
     const int x0 = (tile % a.tiles_x) * TX;
-    const int y0 = ((tile / a.tiles_x) % a.tiles_y) * TY;
-    const int z0 = (tile / (a.tiles_x * a.tiles_y)) * TZ;
+    const int z0 = ((tile / a.tiles_x) % a.tiles_z) * TZ;
+    const int y0 = (tile / (a.tiles_x * a.tiles_z)) * TY;
     const int tid = threadIdx.x;
     const bool mom = tid >= NN;          // role: momentum populations (uniform per wavefront)
     const int nid = mom ? tid - NN : tid;
@@ -113,7 +116,8 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const double* __restrict__ u3_in = a.mac_in + 4 * N;
 
     // ---------------- phase 0: issue the 15 upstream population loads of this
-    // thread's set first, so their latency overlaps staging and stencils
+    // thread's set first, so their latency overlaps staging and stencils; they
+    // are read once, non-temporally, to leave L2 to the macroscopic halos
     const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
     const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
     const bool inside = (x < nx) && (y < ny) && (z < nz);
@@ -131,7 +135,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
         static_for<0, 15>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             using D = Dir<q>;
-            ps[q] = pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]];
+            ps[q] = __builtin_nontemporal_load(&pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]]);
         });
     }
 
