@@ -115,8 +115,36 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const double* __restrict__ u2_in = a.mac_in + 3 * N;
     const double* __restrict__ u3_in = a.mac_in + 4 * N;
 
-    // ---------------- phase 0: issue the 15 upstream population loads of this
-    // thread's set first, so their latency overlaps staging and stencils; they
+    // ---------------- phase 1a: load the macroscopic fields of the tile and its
+    // halos into registers
+    const bool wm = a.wrap_mod != 0;
+    constexpr int IT2 = (H2 + NT - 1) / NT, IT1 = (H1 + NT - 1) / NT;
+    double vphi[IT2], vp[IT1], vu1[IT1], vu2[IT1], vu3[IT1], vtau[IT1];
+    static_for<0, IT2>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        // loads are unconditional (indices clamped into the halo box) so that
+        // the compiler's wait for them is a straight-line count
+        const int h = (tid + k * NT < H2) ? tid + k * NT : H2 - 1;
+        const int hx = h % HX2, hy = (h / HX2) % HY2, hz = h / (HX2 * HY2);
+        const int gx = wrapi(x0 + hx - 2, nx, wm), gy = wrapi(y0 + hy - 2, ny, wm), gz = wrapi(z0 + hz - 2, nz, wm);
+        vphi[k] = phi_in[gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
+    });
+    static_for<0, IT1>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int h = (tid + k * NT < H1) ? tid + k * NT : H1 - 1;
+        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
+        const int gx = wrapi(x0 + hx - 1, nx, wm), gy = wrapi(y0 + hy - 1, ny, wm), gz = wrapi(z0 + hz - 1, nz, wm);
+        const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+        vp[k] = p_in[g];
+        vu1[k] = u1_in[g];
+        vu2[k] = u2_in[g];
+        vu3[k] = u3_in[g];
+        if constexpr (TAUMODE == TAU_FIELD) vtau[k] = a.tau_field[g];
+    });
+
+    // ---------------- phase 1b: issue the 15 upstream population loads of this
+    // thread's set after the field loads, so that the wait for the fields does
+    // not include them and their latency overlaps staging and stencils; they
     // are read once, non-temporally, to leave L2 to the macroscopic halos
     const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
     const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
@@ -130,32 +158,31 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const int64_t nxy = (int64_t)nx * ny;
     const int64_t zo[3] = {nxy * zs[0], nxy * zs[1], nxy * zs[2]};
     const double* __restrict__ pin = mom ? a.g_in : a.f_in;
+    // unconditional: threads outside the grid read node (0, 0, 0)'s neighbours
     double ps[15];
-    if (inside) {
-        static_for<0, 15>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            using D = Dir<q>;
-            ps[q] = __builtin_nontemporal_load(&pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]]);
-        });
-    }
+    static_for<0, 15>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        using D = Dir<q>;
+        ps[q] = __builtin_nontemporal_load(&pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]]);
+    });
 
-    // ---------------- phase 1: stage macroscopic fields
-    const bool wm = a.wrap_mod != 0;
-    for (int h = tid; h < H2; h += NT) {
-        const int hx = h % HX2, hy = (h / HX2) % HY2, hz = h / (HX2 * HY2);
-        const int gx = wrapi(x0 + hx - 2, nx, wm), gy = wrapi(y0 + hy - 2, ny, wm), gz = wrapi(z0 + hz - 2, nz, wm);
-        s_phi[h] = phi_in[gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
-    }
-    for (int h = tid; h < H1; h += NT) {
-        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
-        const int gx = wrapi(x0 + hx - 1, nx, wm), gy = wrapi(y0 + hy - 1, ny, wm), gz = wrapi(z0 + hz - 1, nz, wm);
-        const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-        s_p[h] = p_in[g];
-        s_u1[h] = u1_in[g];
-        s_u2[h] = u2_in[g];
-        s_u3[h] = u3_in[g];
-        if constexpr (TAUMODE == TAU_FIELD) s_tau[h] = a.tau_field[g];
-    }
+    // ---------------- phase 1c: stage the fields in LDS
+    static_for<0, IT2>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int h = tid + k * NT;
+        if (h < H2) s_phi[h] = vphi[k];
+    });
+    static_for<0, IT1>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int h = tid + k * NT;
+        if (h < H1) {
+            s_p[h] = vp[k];
+            s_u1[h] = vu1[k];
+            s_u2[h] = vu2[k];
+            s_u3[h] = vu3[k];
+            if constexpr (TAUMODE == TAU_FIELD) s_tau[h] = vtau[k];
+        }
+    });
     __syncthreads();
 
     // ---------------- phase 2: stencils, F-bracket and tau at tile + 1 halo
