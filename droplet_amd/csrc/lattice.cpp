@@ -245,6 +245,7 @@ void lattice_free(drp_lattice* L) {
     if (L->tg) (void)hipFree(L->tg);
     if (L->partial) (void)hipFree(L->partial);
     if (L->mac64) (void)hipFree(L->mac64);
+    if (L->census) (void)hipFree(L->census);
     if (L->stream && L->owns_stream) (void)hipStreamDestroy(L->stream);
     L->stream = nullptr;
 }

@@ -31,6 +31,7 @@ struct drp_lattice {
     double* partial = nullptr;          // diagnostics partial sums
     int64_t partial_cap = 0;
     double* mac64 = nullptr;            // FP32 mode: FP64 copy of the macroscopic fields
+    int* census = nullptr;              // droplet census work space (4 n + 4 ints, census.hip)
     void* fp32_state = nullptr;         // FP32 mode: shift constants etc.
 };
 

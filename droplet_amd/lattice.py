@@ -66,6 +66,9 @@ def lib():
         L.drp_autotune.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(C.c_int), dp]
         L.drp_copy_bandwidth.argtypes = [C.c_size_t, C.c_int, dp]
         L.drp_copy_bandwidth_soa.argtypes = [C.c_size_t, C.c_int, C.c_int, dp]
+        i64p = C.POINTER(C.c_int64)
+        L.drp_droplet_census.argtypes = [vp, C.c_double, i64p, i64p, i64p, C.c_int64, C.POINTER(C.c_int32),
+                                         C.c_size_t]
         L.drp_batch_create.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(vp)]
         L.drp_batch_size.argtypes = [vp]
         L.drp_batch_member.argtypes = [vp, C.c_int]
@@ -217,6 +220,25 @@ class Lattice:
         L, B, D = C.c_double(), C.c_double(), C.c_double()
         _check(lib().drp_deformation(self._h, int(mode), C.byref(L), C.byref(B), C.byref(D)), "drp_deformation")
         return L.value, B.value, D.value
+
+    def droplet_census(self, threshold: float | None = None, cap: int = 1 << 16, labels: bool = False):
+        """Droplets of the current state (drp_droplet_census): (first, volumes[, labels]),
+        ascending in first; threshold defaults to (phi1 + phi2) / 2 as in
+        program/percolation.f90."""
+        thr = 0.5 * (self.cfg.phi1 + self.cfg.phi2) if threshold is None else float(threshold)
+        cnt = C.c_int64(0)
+        vol = np.zeros(cap, np.int64)
+        first = np.zeros(cap, np.int64)
+        lab = np.zeros(self.shape, np.int32) if labels else None
+        i64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))  # noqa: E731
+        _check(lib().drp_droplet_census(self._h, thr, C.byref(cnt), i64(vol), i64(first), cap,
+                                        lab.ctypes.data_as(C.POINTER(C.c_int32)) if labels else None,
+                                        self.n if labels else 0), "drp_droplet_census")
+        k = cnt.value
+        if k > cap:
+            raise ValueError(f"{k} droplets exceed cap = {cap}")
+        out = (first[:k].copy(), vol[:k].copy())
+        return out + (lab,) if labels else out
 
 
 def tile_shape_count() -> int:

@@ -275,6 +275,20 @@ int drp_copy_bandwidth(size_t bytes, int reps, double* gbs);
  * kernel's access pattern without its arithmetic. */
 int drp_copy_bandwidth_soa(size_t bytes_per_stream, int streams, int reps, double* gbs);
 
+/* Droplet census (program/percolation.f90:31-349, the offline Hoshen-Kopelman
+   droplet count of a field file; there it reads phi, labels clusters of
+   phi >= (phi1 + phi2) / 2 and writes their number).  Here on the current
+   state on the GPU: clusters of nodes with phi >= threshold connected through
+   face neighbours, periodic in x, y and z.  count: number of clusters;
+   volumes[k] (nodes) and first[k] (smallest node index x + nx (y + ny z) in the
+   cluster) for k < min(count, cap), ordered by first; labels (nullable,
+   n = nodes): per node the first index of its cluster, -1 outside.  The
+   reference's scan links 5 of the 12 edge neighbours as well and follows
+   periodic wraps only partly; on clusters that neither touch diagonally nor
+   cross a wrap the counts agree.  Needs fewer than 2^30 nodes. */
+int drp_droplet_census(drp_lattice* lat, double threshold, int64_t* count, int64_t* volumes, int64_t* first,
+                       int64_t cap, int32_t* labels, size_t n);
+
 /* Message for the calling thread's last failure ("" if none). */
 const char* drp_last_error(void);
 

@@ -27,7 +27,8 @@ def build_planted(plant: int, out_dir: str) -> str:
     """Compile a copy of the restatement with one planted mistake (see ORACLE_PLANT)."""
     out = os.path.join(out_dir, f"liblbm_oracle_plant{plant}.so")
     subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c11", f"-DORACLE_PLANT={plant}",
-                    "-shared", "-o", out, os.path.join(HERE, "lbm_oracle.c"), "-lm"], check=True)
+                    "-shared", "-o", out, os.path.join(HERE, "lbm_oracle.c"), os.path.join(HERE, "census_oracle.c"),
+                    "-lm"], check=True)
     return out
 
 
@@ -67,7 +68,40 @@ def _bind(L):
     L.ora_get_force.argtypes = [vp, dp, dp, dp]
     L.ora_spectral_force_fields.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.ora_spectral_force_fields.restype = C.c_double
+    i64p, i32p = C.POINTER(C.c_int64), C.POINTER(C.c_int32)
+    L.ora_census.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_double, i64p, i64p, i64p, C.c_int64, i32p]
+    L.ora_percolation_count.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_double, i64p]
     return L
+
+
+def census(phi: np.ndarray, threshold: float):
+    """Face-connected periodic clusters of phi >= threshold (phi [z][y][x]):
+    (first, volumes, labels) with first / volumes ascending in first."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    nz, ny, nx = phi.shape
+    n = phi.size
+    cnt = C.c_int64(0)
+    vol = np.zeros(n, np.int64)
+    first = np.zeros(n, np.int64)
+    lab = np.zeros(phi.shape, np.int32)
+    i64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))  # noqa: E731
+    rc = lib().ora_census(_dp(phi), nx, ny, nz, threshold, C.byref(cnt), i64(vol), i64(first), n,
+                          lab.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc:
+        raise MemoryError("ora_census")
+    k = cnt.value
+    return first[:k].copy(), vol[:k].copy(), lab
+
+
+def percolation_count(phi: np.ndarray, threshold: float) -> int:
+    """program/percolation.f90's droplet count of one field (phi [z][y][x])."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    nz, ny, nx = phi.shape
+    out = C.c_int64(0)
+    rc = lib().ora_percolation_count(_dp(phi), nx, ny, nz, threshold, C.byref(out))
+    if rc:
+        raise RuntimeError("percolation scan needs more than 1000 provisional labels" if rc == -2 else "ora_percolation_count")
+    return out.value
 
 
 def _dp(a: np.ndarray):

@@ -36,7 +36,7 @@ E = np.array([2 / 9] + [1 / 9] * 6 + [1 / 72] * 8)
 
 # ------------------------------------------------------------------ Fortran I/O
 from fortran_io import (assemble_periodic, assemble_pops, assemble_xz, read_records, write_fg_input,  # noqa: E402
-                        write_g_input)
+                        write_g_input, write_records)
 
 
 def fnum(v: float) -> str:
@@ -197,6 +197,56 @@ def case_les(name, N, steps, step_input, seed, delta, Cs, Re, Dv, D, sigma, umax
     return data, meta
 
 
+def sphere_field(N, drops, phi1=2.211, phi2=4.895):
+    """Sharp periodic spheres: phi2 where the minimum-image distance to a centre
+    (xc, yc, zc) is <= r, phi1 elsewhere; [z][y][x].  Pure float64 arithmetic,
+    so the tests rebuild the same fields from the stored lists."""
+    z, y, x = np.meshgrid(np.arange(N, dtype=np.float64), np.arange(N, dtype=np.float64),
+                          np.arange(N, dtype=np.float64), indexing="ij")
+    phi = np.full((N, N, N), phi1)
+    for xc, yc, zc, r in drops:
+        dx = np.abs(x - xc)
+        dy = np.abs(y - yc)
+        dz = np.abs(z - zc)
+        dx = np.minimum(dx, N - dx)
+        dy = np.minimum(dy, N - dy)
+        dz = np.minimum(dz, N - dz)
+        phi[dx * dx + dy * dy + dz * dz <= r * r] = phi2
+    return phi
+
+
+def case_dsd(name, N, seed):
+    """program/percolation.f90 on 71 fields (its loop reads 1_<step>.bin for step =
+    5000, 6000, ..., 75000 and appends (step, droplet count) to drop_num_1.d):
+    1-12 sharp spheres of radius 1.5-5 at seeded positions, periodic images
+    included, so some cross the periodic seams and some overlap."""
+    params = dict(xmax=str(N - 1), ymax=str(N - 1), zmax=str(N - 1), phi1="2.211d0", phi2="4.895d0",
+                  datadir='"./"', datadir2='"./radius/"')
+    exe = ref_build.build("program/percolation.f90", f"percolation_{name}", params)
+    rng = np.random.default_rng(seed)
+    steps = list(range(5000, 75001, 1000))
+    drops = np.zeros((len(steps), 12, 4))
+    ndrops = np.zeros(len(steps), np.int64)
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "radius"))
+        for k, st in enumerate(steps):
+            nd = 1 + k % 12
+            ndrops[k] = nd
+            drops[k, :nd, :3] = rng.uniform(0.0, N, (nd, 3))
+            drops[k, :nd, 3] = rng.choice([1.5, 2.0, 2.6, 3.2, 4.1, 5.0], nd)
+            phi = sphere_field(N, drops[k, :nd])
+            # one record per node, z outermost, x fastest (:50-58)
+            write_records(os.path.join(tmp, f"1_{st}.bin"), phi.ravel())
+        subprocess.run([exe], cwd=tmp, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+        rows = np.loadtxt(os.path.join(tmp, "drop_num_1.d"))
+    assert rows.shape == (len(steps), 2) and np.array_equal(rows[:, 0], steps)
+    data = {"drops": drops, "ndrops": ndrops, "counts": rows[:, 1].astype(np.int64)}
+    meta = dict(case=name, program="program/percolation.f90", params=params, N=N, steps=steps,
+                phi1=2.211, phi2=4.895,
+                sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "program/percolation.f90")))
+    return data, meta
+
+
 def save(name, data, meta):
     path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, meta=np.array(json.dumps(meta)), **data)
@@ -218,6 +268,8 @@ CASES = {
     # deformation measure (:905-944) never exits
     "walls24": lambda: case_walls("walls24", 24, 30000, D=12.0, nu=0.1, uw=0.01, kappag=0.01),
     # Smagorinsky LES (Cs 0.173, Delta 4) with a droplet, injection + Taylor-Green force, 500 steps
+    # droplet counts of program/percolation.f90 for 71 seeded sphere layouts at 32^3
+    "dsd32": lambda: case_dsd("dsd32", 32, seed=31),
     "les16": lambda: case_les("les16", 16, 500, step_input=3, seed=21, delta=4.0, Cs=0.173, Re=80.0, Dv=8.0,
                               D=8.0, sigma=1e-4),
 }
