@@ -135,6 +135,19 @@ module droplet_amd
             real(c_double), intent(out) :: fx(*), fy(*), fz(*)
             integer(c_size_t), value :: n
         end function
+        ! droplet census of the current state (the count of program/percolation.f90):
+        ! labels = c_null_ptr or c_loc of an integer(c_int32_t) array of n nodes
+        integer(c_int) function drp_droplet_census(lat, threshold, count, volumes, first, cap, labels, n) &
+                bind(C, name="drp_droplet_census")
+            import :: c_int, c_ptr, c_double, c_int64_t, c_size_t
+            type(c_ptr), value :: lat
+            real(c_double), value :: threshold
+            integer(c_int64_t), intent(out) :: count
+            integer(c_int64_t), intent(out) :: volumes(*), first(*)
+            integer(c_int64_t), value :: cap
+            type(c_ptr), value :: labels
+            integer(c_size_t), value :: n
+        end function
         ! output / outputfg / step_input files in the reference's per-rank layout
         integer(c_int) function drp_write_reference_files_c(lat, dir, n, ranks_x, ranks_y, kind) &
                 bind(C, name="drp_write_reference_files")

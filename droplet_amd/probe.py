@@ -99,9 +99,31 @@ def turbulence(a, dl):
                       "kinetic": st["kinetic"], "phi_mass": st["phi_mass"], "host_init_s": round(t_init, 1)}))
 
 
+def droplet_census(a, dl):
+    """Census of the turbulent-droplet state (32 droplets of radius 10) after a
+    few steps: droplet count, volume range and time per call (HIP sync included)."""
+    import time
+    from droplet_amd import hit
+    from droplet_amd.abi import DRP_FP32_SHIFTED, DRP_FP64
+    N = a.grid
+    phi, p, u1, u2, u3, _ = hit.initial_state(N, seed=3, u_rms=0.03, count=32, D=20.0)
+    cfg = hit.hit_config(N, We=a.we, D=20.0, precision=DRP_FP32_SHIFTED if a.fp32 else DRP_FP64)
+    lat = dl.Lattice(cfg)
+    lat.init_fields(phi, p, u1, u2, u3)
+    del phi, p, u1, u2, u3
+    lat.step(10)
+    lat.droplet_census()
+    t0 = time.perf_counter()
+    for _ in range(a.reps):
+        first, vol = lat.droplet_census()
+    ms = (time.perf_counter() - t0) * 1e3 / a.reps
+    print(json.dumps({"grid": N, "droplets": int(len(first)), "volume_min": int(vol.min()), "volume_max": int(vol.max()),
+                      "ms_per_call": ms, "reps": a.reps}))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "copysoa", "step", "autotune", "ensemble", "hit"])
+    ap.add_argument("what", choices=["copy", "copysoa", "step", "autotune", "ensemble", "hit", "census"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
@@ -118,6 +140,9 @@ def main(argv=None):
         return
     if a.what == "hit":
         turbulence(a, dl)
+        return
+    if a.what == "census":
+        droplet_census(a, dl)
         return
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
