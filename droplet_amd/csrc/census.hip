@@ -9,7 +9,7 @@
 //               (union by atomicMin: a root only ever gains a smaller parent,
 //               so every root is the smallest node index of its cluster and
 //               parent chains strictly decrease)
-//   cc_flatten  label[i] = root, volume[root] += 1
+//   cc_flatten  label[i] = root, volume[root] += 1 (one atomic per root and wavefront)
 //   cc_collect  one (root, volume) record per cluster, in any order; the host
 //               sorts them by root
 // Work buffer (int32, 4 n + 4): labels | volumes | records (2 per cluster) |
@@ -87,12 +87,27 @@ __global__ __launch_bounds__(256) void cc_union(int nx, int ny, int nz, int* lab
 
 __global__ __launch_bounds__(256) void cc_flatten(int n, int* lab, int* __restrict__ vol) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || lab[i] < 0) return;
-    const int r = find_root(lab, i);
-    atomicAdd(&vol[r], 1);
-    // only the own entry is written; other threads read it as a valid parent
-    // (r is an ancestor of i), so their finds stay correct
-    __atomic_store_n(&lab[i], r, __ATOMIC_RELAXED);
+    int r = -1;
+    if (i < n && lab[i] >= 0) {
+        r = find_root(lab, i);
+        // only the own entry is written; other threads read it as a valid
+        // parent (r is an ancestor of i), so their finds stay correct
+        __atomic_store_n(&lab[i], r, __ATOMIC_RELAXED);
+    }
+    // volume counts: one atomic per distinct root of the wavefront (neighbouring
+    // nodes mostly share a root), not one per node
+    const int lane = threadIdx.x & 63;
+    bool pending = r >= 0;
+    for (;;) {
+        const unsigned long long m = __ballot(pending);
+        if (m == 0) break;
+        const int leader = __ffsll((long long)m) - 1;
+        const int lr = __shfl(r, leader, 64);
+        const bool same = pending && r == lr;
+        const unsigned long long sm = __ballot(same);
+        if (lane == leader) atomicAdd(&vol[lr], __popcll(sm));
+        pending = pending && !same;
+    }
 }
 
 __global__ __launch_bounds__(256) void cc_collect(int n, const int* __restrict__ lab, const int* __restrict__ vol,
