@@ -63,10 +63,15 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     // banks).  56 bytes per halo node: two 1024-thread workgroups fit one CU.
     // For a diagonal c (c_a^2 = 1) the contraction 4.5 (c.g)^2 - 1.5 |g|^2 |c|^2
     // reduces to 9 (cx cy g0 g1 + cx cz g0 g2 + cy cz g1 g2).
-    __shared__ float s_phi[H2];  // phit
+    __shared__ float s_phi[H2];  // phit; with a tau field its first H1 entries hold 1 / tau_g in phase 3
     __shared__ float4 s_fa[H1], s_ga[H1];
     __shared__ float s_k[6][H1];
-    __shared__ float s_rtau[TAUMODE == 0 ? 1 : H1];  // 1 / tau_g
+    // 1 / tau_g per halo node reuses s_phi once the stencils are done, which keeps
+    // the variable-tau variants at two workgroups per CU (an own array would add
+    // 4 B per halo node and exceed half of the LDS)
+    float* const s_rtau = s_phi;
+    constexpr int IT1 = (H1 + NT - 1) / NT;
+    float rt_reg[TAUMODE == 0 ? 1 : IT1];
     __shared__ double s_red[2][NT / 64];
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -112,13 +117,18 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         const int gx = wrap32(x0 + hx - 2, nx, wm), gy = wrap32(y0 + hy - 2, ny, wm), gz = wrap32(z0 + hz - 2, nz, wm);
         s_phi[h] = phi_in[gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
     }
-    for (int h = tid; h < H1; h += NT) {
-        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
-        const int gx = wrap32(x0 + hx - 1, nx, wm), gy = wrap32(y0 + hy - 1, ny, wm), gz = wrap32(z0 + hz - 1, nz, wm);
-        const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-        s_ga[h] = make_float4(p_in[g], u1_in[g], u2_in[g], u3_in[g]);
-        if constexpr (TAUMODE == 2) s_rtau[h] = 1.0f / a.tau_field[g];
-    }
+    sfor32<0, IT1>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int h = tid + k * NT;
+        if (h < H1) {
+            const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
+            const int gx = wrap32(x0 + hx - 1, nx, wm), gy = wrap32(y0 + hy - 1, ny, wm),
+                      gz = wrap32(z0 + hz - 1, nz, wm);
+            const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+            s_ga[h] = make_float4(p_in[g], u1_in[g], u2_in[g], u3_in[g]);
+            if constexpr (TAUMODE == 2) rt_reg[k] = 1.0f / a.tau_field[g];
+        }
+    });
     __syncthreads();
 
     // phase 2: FP64-accumulated Laplacian and gradient, deviation F-bracket, the
@@ -126,7 +136,10 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     const double rmax = from_ordered_bits(a.range[2 * a.slot_read + 0]);
     const double rmin = from_ordered_bits(a.range[2 * a.slot_read + 1]);
     const float range = (float)(rmax - rmin), rminf = (float)(rmin - a.phi1);
-    for (int h = tid; h < H1; h += NT) {
+    sfor32<0, IT1>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int h = tid + k * NT;
+        if (h >= H1) return;
         const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
         const int c = (hx + 1) + HX2 * ((hy + 1) + HY2 * (hz + 1));
         const float pt = s_phi[c];
@@ -160,10 +173,22 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
         s_k[5][h] = 9.0f * gf1 * gf2;
         if constexpr (TAUMODE == 1) {
             float frac = (range == 0.0f) ? 0.0f : (pt - rminf) / range;
-            s_rtau[h] = 1.0f / (0.5f + 3.0f * (frac * a.dnuf + a.nu1f));
+            rt_reg[k] = 1.0f / (0.5f + 3.0f * (frac * a.dnuf + a.nu1f));
         }
-    }
+    });
+    // with a tau field, the own node's phit (the f-role's q = 0 term) is read
+    // before s_phi is reused
+    const int h2self = (tx + 2) + HX2 * ((ty + 2) + HY2 * (tz + 2));
+    float phi_self = 0.0f;
+    if constexpr (TAUMODE != 0) phi_self = s_phi[h2self];
     __syncthreads();
+    if constexpr (TAUMODE != 0) {
+        sfor32<0, IT1>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if (tid + k * NT < H1) s_rtau[tid + k * NT] = rt_reg[k];
+        });
+        __syncthreads();
+    }
 
     // Korteweg contraction gt_q and c_q . v from the records of the upstream node
     auto gt_of = [&](auto qc, int hu) -> float {
@@ -196,8 +221,7 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
                 const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
                 float fe;
                 if constexpr (q == 0) {
-                    const int h2 = (tx + 2) + HX2 * ((ty + 2) + HY2 * (tz + 2));
-                    fe = __builtin_fmaf(Fw, s_fa[hself].x, s_phi[h2]);
+                    fe = __builtin_fmaf(Fw, s_fa[hself].x, TAUMODE != 0 ? phi_self : s_phi[h2self]);
                 } else {
                     const float4 fa = s_fa[hu];
                     const float cv = cdot<q>(fa.y, fa.z, fa.w);

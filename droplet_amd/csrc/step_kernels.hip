@@ -259,6 +259,21 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             vmax = ph;
             vmin = ph;
         } else {
+            // force-field components of the upstream nodes, loaded FPF directions
+            // ahead of their use into a ring of FPF slots, so that the L2 latency
+            // of these loads overlaps the collisions in between
+            constexpr int FPF = 3;
+            double fring[FPF][3] = {};
+            auto load_force = [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
+                double* f = fring[q % FPF];
+                if constexpr (D::cx != 0) f[0] = a.fx[src];
+                if constexpr (D::cy != 0) f[1] = a.fy[src];
+                if constexpr (D::cz != 0) f[2] = a.fz[src];
+            };
+            if constexpr (FORCE == FORCE_FIELD) static_for<1, 1 + FPF>(load_force);
             static_for<0, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
@@ -289,11 +304,12 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                         const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
                         if constexpr (D::cx != 0) Fx = a.tg_amp * a.sinx[xu] * a.cosz[zu];
                         if constexpr (D::cz != 0) Fz = -(a.tg_amp * a.cosx[xu] * a.sinz[zu]);
-                    } else {  // static field, read at the upstream node through L2
-                        const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
-                        if constexpr (D::cx != 0) Fx = a.fx[src];
-                        if constexpr (D::cy != 0) Fy = a.fy[src];
-                        if constexpr (D::cz != 0) Fz = a.fz[src];
+                    } else {  // field at the upstream node, loaded FPF directions ahead
+                        const double* f = fring[q % FPF];
+                        if constexpr (D::cx != 0) Fx = f[0];
+                        if constexpr (D::cy != 0) Fy = f[1];
+                        if constexpr (D::cz != 0) Fz = f[2];
+                        if constexpr (q + FPF < 15) load_force(std::integral_constant<int, q + FPF>{});
                     }
                     const double dot = cdot<q>(Fx, Fy, Fz);
                     gv = gv + 3.0 * weightE<q>() * dot;
