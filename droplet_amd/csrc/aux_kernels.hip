@@ -132,25 +132,58 @@ __global__ void moments_fp64(int64_t N, const double* __restrict__ f, const doub
 // nu_e = Cs^2 Delta^2 sqrt(2 S:S) with S_ab = 0.5 (d_b u_a + d_a u_b) (strain_cal
 // :479-499, :661-677; amdflang lowers **0.5d0 to sqrt) and tau = 0.5 + 3 (nu + nu_e)
 // (:679-690).  mac_old == mac gives the tau of a state with current halos (ini_op).
-__global__ void les_tau_fp64(Grid G, Phys P, const double* __restrict__ mac, const double* __restrict__ mac_old,
-                             LesBlocks B, const uint64_t* __restrict__ range, int slot, double* __restrict__ tau) {
+// One workgroup per LES_TX x LES_TY x LES_TZ tile: u (and, with rank blocks, the
+// old u) of the tile and its 1-node halo staged in LDS, one node per thread.
+constexpr int LES_TX = 32, LES_TY = 4, LES_TZ = 2;
+// periodic index of v in [-1, n + tile]: one conditional add or subtract, or a
+// full modulo where a tile is longer than the grid (small)
+__device__ __forceinline__ int wrap_any(int v, int n, bool small) {
+    if (small) return ((v % n) + n) % n;
+    return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+
+template <bool STALE>
+__global__ __launch_bounds__(LES_TX* LES_TY* LES_TZ) void les_tau_fp64(Grid G, Phys P, const double* __restrict__ mac,
+                                                                       const double* __restrict__ mac_old, LesBlocks B,
+                                                                       const uint64_t* __restrict__ range, int slot,
+                                                                       double* __restrict__ tau, int tiles_x,
+                                                                       int tiles_y) {
+    constexpr int HX = LES_TX + 2, HY = LES_TY + 2, HZ = LES_TZ + 2, H = HX * HY * HZ, NT = LES_TX * LES_TY * LES_TZ;
+    __shared__ double su[3][H];
+    __shared__ double so[STALE ? 3 : 1][STALE ? H : 1];
     const int64_t N = (int64_t)G.nx * G.ny * G.nz;
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (s >= N) return;
-    const int x = s % G.nx, y = (s / G.nx) % G.ny, z = s / ((int64_t)G.nx * G.ny);
+    const int tile = blockIdx.x;
+    const int x0 = (tile % tiles_x) * LES_TX, y0 = ((tile / tiles_x) % tiles_y) * LES_TY,
+              z0 = (tile / (tiles_x * tiles_y)) * LES_TZ;
+    const bool small = G.nx <= LES_TX || G.ny <= LES_TY || G.nz <= LES_TZ;
+    for (int h = threadIdx.x; h < H; h += NT) {
+        const int hx = h % HX, hy = (h / HX) % HY, hz = h / (HX * HY);
+        const int64_t g = wrap_any(x0 + hx - 1, G.nx, small) +
+                          (int64_t)G.nx * (wrap_any(y0 + hy - 1, G.ny, small) +
+                                           (int64_t)G.ny * wrap_any(z0 + hz - 1, G.nz, small));
+        for (int k = 0; k < 3; k++) {
+            su[k][h] = mac[(2 + k) * N + g];
+            if constexpr (STALE) so[k][h] = mac_old[(2 + k) * N + g];
+        }
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % LES_TX, ty = (threadIdx.x / LES_TX) % LES_TY, tz = threadIdx.x / (LES_TX * LES_TY);
+    const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
+    if (x >= G.nx || y >= G.ny || z >= G.nz) return;
+    const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
     double gu[3][3];
     for (int k = 0; k < 3; k++)
         for (int a = 0; a < 3; a++) gu[k][a] = 0.0;
     sfor<1, 15>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         using D = Dir<q>;
-        const int64_t t = G.at(x + D::cx, y + D::cy, z + D::cz);
-        const double* u = les_halo(B, x, y, z, x + D::cx, y + D::cy, z + D::cz) ? mac_old : mac;
-        const double v[3] = {u[2 * N + t], u[3 * N + t], u[4 * N + t]};
+        const int h = (tx + 1 + D::cx) + HX * ((ty + 1 + D::cy) + HY * (tz + 1 + D::cz));
+        const bool old = STALE && les_halo(B, x, y, z, x + D::cx, y + D::cy, z + D::cz);
         for (int k = 0; k < 3; k++) {
-            gu[k][0] = acc<D::cx>(gu[k][0], v[k]);
-            gu[k][1] = acc<D::cy>(gu[k][1], v[k]);
-            gu[k][2] = acc<D::cz>(gu[k][2], v[k]);
+            const double v = old ? so[STALE ? k : 0][STALE ? h : 0] : su[k][h];
+            gu[k][0] = acc<D::cx>(gu[k][0], v);
+            gu[k][1] = acc<D::cy>(gu[k][1], v);
+            gu[k][2] = acc<D::cz>(gu[k][2], v);
         }
     });
     for (int k = 0; k < 3; k++)
@@ -243,9 +276,12 @@ hipError_t launch_moments_fp64(int nx, int ny, int nz, const double* f, const do
 hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const double* mac_old,
                                const LesBlocks& blk, const uint64_t* range, int slot, double* tau, hipStream_t st) {
     Grid G{nx, ny, nz};
-    const int64_t N = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(les_tau_fp64, dim3(nblk(N, 256)), dim3(256), 0, st, G, ph, mac, mac_old, blk, range, slot,
-                       tau);
+    const int tx = (nx + LES_TX - 1) / LES_TX, ty = (ny + LES_TY - 1) / LES_TY, tz = (nz + LES_TZ - 1) / LES_TZ;
+    const dim3 grid((unsigned)(tx * ty * tz)), block(LES_TX * LES_TY * LES_TZ);
+    if (blk.bx > 0 && mac_old != mac)
+        hipLaunchKernelGGL(les_tau_fp64<true>, grid, block, 0, st, G, ph, mac, mac_old, blk, range, slot, tau, tx, ty);
+    else
+        hipLaunchKernelGGL(les_tau_fp64<false>, grid, block, 0, st, G, ph, mac, mac_old, blk, range, slot, tau, tx, ty);
     return hipGetLastError();
 }
 
