@@ -39,7 +39,7 @@ def ensemble(a, dl):
     sigma = 1.7039 * kappa_g ** 0.9991
     cfgs, drops = [], []
     for _ in range(a.members):
-        N = int(rng.choice([48, 64, 96, 128]))
+        N = int(rng.choice(a.sizes))
         Ca = float(rng.uniform(0.05, 0.5))
         D = 2.0 * N / 5.0
         U = Ca * (N - 1) * sigma / (0.1 * D)   # Ca = U nu D / (H sigma)
@@ -131,6 +131,8 @@ def main(argv=None):
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--shape", type=int, default=-1)
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--sizes", type=lambda s: [int(v) for v in s.split(",")], default=[48, 64, 96, 128],
+                    help="ensemble member grid sizes to draw from (comma separated)")
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--we", type=float, default=5.0)
     a = ap.parse_args(argv)
