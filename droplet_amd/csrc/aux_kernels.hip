@@ -125,6 +125,15 @@ __global__ void moments_fp64(int64_t N, const double* __restrict__ f, const doub
     }
 }
 
+// LES tau tiles of LES_TX x LES_TY x LES_TZ nodes, one node per thread
+constexpr int LES_TX = 32, LES_TY = 4, LES_TZ = 2;
+// periodic index of v in [-1, n + tile): one conditional add or subtract, or a
+// full modulo where a tile is longer than the grid (small)
+__device__ __forceinline__ int wrap_any(int v, int n, bool small) {
+    if (small) return ((v % n) + n) % n;
+    return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+
 // tau_g for the next step from the state a step left (mac: phi, p, u of this
 // step; mac_old: the state the step started from) and that step's phi range:
 // grad u as grad_u_cal takes it at the end of physics (:447-477, :660), with
@@ -132,16 +141,8 @@ __global__ void moments_fp64(int64_t N, const double* __restrict__ f, const doub
 // nu_e = Cs^2 Delta^2 sqrt(2 S:S) with S_ab = 0.5 (d_b u_a + d_a u_b) (strain_cal
 // :479-499, :661-677; amdflang lowers **0.5d0 to sqrt) and tau = 0.5 + 3 (nu + nu_e)
 // (:679-690).  mac_old == mac gives the tau of a state with current halos (ini_op).
-// One workgroup per LES_TX x LES_TY x LES_TZ tile: u (and, with rank blocks, the
-// old u) of the tile and its 1-node halo staged in LDS, one node per thread.
-constexpr int LES_TX = 32, LES_TY = 4, LES_TZ = 2;
-// periodic index of v in [-1, n + tile]: one conditional add or subtract, or a
-// full modulo where a tile is longer than the grid (small)
-__device__ __forceinline__ int wrap_any(int v, int n, bool small) {
-    if (small) return ((v % n) + n) % n;
-    return v < 0 ? v + n : (v >= n ? v - n : v);
-}
-
+// u (and, with rank blocks, the old u) of a tile and its 1-node halo are staged
+// in LDS once per workgroup.
 template <bool STALE>
 __global__ __launch_bounds__(LES_TX* LES_TY* LES_TZ) void les_tau_fp64(Grid G, Phys P, const double* __restrict__ mac,
                                                                        const double* __restrict__ mac_old, LesBlocks B,
