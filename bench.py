@@ -89,6 +89,9 @@ def cpu_reference_mlups():
     t = {int(float(a[0])): float(a[1]) for a in rows}
     if 100 not in t or 300 not in t:
         return None
+    # the third column is the run's u1 at one node: a run that went non-finite is not timed
+    if not all(len(a) > 2 and math.isfinite(float(a[2].replace("D", "E"))) for a in rows):
+        return None
     mlups = 64 ** 3 * 200 / (t[300] - t[100]) / 1e6
     return {"value": round(mlups, 3), "unit": "MLUPS", "cores": 8, "kind": "reference",
             "sample": "program/lbm_drop_xy.f90 compiled from source (amdflang -O2, MPICH), 64^3 periodic "
