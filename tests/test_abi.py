@@ -60,3 +60,21 @@ def test_invalid_config_is_rejected_without_gpu_work():
     c = abi.make_config(16, anu=0.1)
     assert lib.drp_create(C.byref(c), C.byref(h)) == abi.DRP_ERR_UNSUPPORTED
     assert lib.drp_step(None, 1) == abi.DRP_ERR_ARG
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(les_ranks_x=8, les_ranks_y=0, les_ranks_z=2), b"all 0 or all positive"),
+    (dict(les_ranks_x=3, les_ranks_y=1, les_ranks_z=1), b"do not divide"),
+    (dict(les_ranks_x=-1, les_ranks_y=-1, les_ranks_z=-1), b"all 0 or all positive"),
+    (dict(reserved=(C.c_int32 * 5)(0, 0, 0, 0, 7)), b"reserved"),
+    (dict(tau_f=0.5), b"tau_f"),
+])
+def test_config_fields_are_validated_without_gpu_work(kw, msg):
+    lib = C.CDLL(LIB)
+    lib.drp_last_error.restype = C.c_char_p
+    c = abi.make_config(16, les_cs=0.1, les_delta=1.0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    h = C.c_void_p()
+    assert lib.drp_create(C.byref(c), C.byref(h)) == abi.DRP_ERR_ARG and not h.value
+    assert msg in lib.drp_last_error()
