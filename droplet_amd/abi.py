@@ -27,7 +27,8 @@ DRP_FORCE_NONE = 0
 DRP_FORCE_TAYLOR_GREEN = 1
 DRP_FORCE_FIELD = 2
 DRP_FORCE_SPECTRAL = 3
-DRP_FILE_FIELDS, DRP_FILE_FG, DRP_FILE_G = 0, 1, 2
+DRP_FILE_FIELDS, DRP_FILE_FG, DRP_FILE_G, DRP_FILE_FG_G = 0, 1, 2, 3
+DRP_FILE_LAYOUT_XZ = 0x100
 
 FIELD_PHI, FIELD_P, FIELD_U1, FIELD_U2, FIELD_U3 = range(5)
 FIELDS = {"phi": FIELD_PHI, "p": FIELD_P, "u1": FIELD_U1, "u2": FIELD_U2, "u3": FIELD_U3}

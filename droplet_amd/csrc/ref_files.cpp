@@ -7,10 +7,14 @@
 //             (:721-743); read back for restarts (program/LBM.f90:371-562)
 //   g input   <dir>/0_<rank>_<n>_fg.bin   one record g_i per direction and node, the
 //             momentum populations injected at step_input (:789-828)
+//   (f,g) input of program/LES_drop.f90 (:919-945): outputfg records, g kept
 // Within a file the loops run z outermost, then y, then x (then i); rank r holds
 // the block x in [rx xp, (rx+1) xp), y in [ry yp, (ry+1) yp), all z, with
 // rx = r / ranks_y, ry = r % ranks_y, xp = nx / ranks_x, yp = ny / ranks_y
-// (comm_rank_x = comm_rank / Ny, :145-149).
+// (comm_rank_x = comm_rank / Ny, :145-149).  With DRP_FILE_LAYOUT_XZ the split is
+// program/LES_drop.f90's (:153-176): x in [ix xp, ...), z in [iz zp, ...), all y,
+// ix = r % ranks_x, iz = r / ranks_x, zp = nz / ranks_y (the second count is the
+// number of z slabs there).
 #include <hip/hip_runtime.h>
 
 #include <cerrno>
@@ -31,18 +35,24 @@ std::string file_name(const char* dir, int rank, int64_t n, bool fg) {
 }
 
 struct Block {
-    int x0, y0, xp, yp;
+    int x0, y0, z0, xp, yp, zp;
 };
 
-int check_split(const drp_config& c, int rx, int ry) {
+int check_split(const drp_config& c, int rx, int ry, bool xz) {
     if (rx < 1 || ry < 1) return drp_fail(DRP_ERR_ARG, "rank counts must be positive");
-    if (c.nx % rx || c.ny % ry) return drp_fail(DRP_ERR_ARG, "%dx%d ranks do not divide %dx%d", rx, ry, c.nx, c.ny);
+    const int second = xz ? c.nz : c.ny;
+    if (c.nx % rx || second % ry)
+        return drp_fail(DRP_ERR_ARG, "%dx%d ranks do not divide %dx%d (x, %s)", rx, ry, c.nx, second, xz ? "z" : "y");
     return DRP_OK;
 }
 
-Block block_of(const drp_config& c, int rank, int rx, int ry) {
+Block block_of(const drp_config& c, int rank, int rx, int ry, bool xz) {
+    if (xz) {
+        const int xp = c.nx / rx, zp = c.nz / ry;
+        return Block{(rank % rx) * xp, 0, (rank / rx) * zp, xp, c.ny, zp};
+    }
     const int xp = c.nx / rx, yp = c.ny / ry;
-    return Block{(rank / ry) * xp, (rank % ry) * yp, xp, yp};
+    return Block{(rank / ry) * xp, (rank % ry) * yp, 0, xp, yp, c.nz};
 }
 
 // one file of equal records of nval doubles, written / read in loop order
@@ -70,9 +80,11 @@ struct RecordFile {
 extern "C" int drp_write_reference_files(drp_lattice* L, const char* dir, int64_t n, int ranks_x, int ranks_y,
                                          int kind) {
     if (!L || !dir) return drp_fail(DRP_ERR_ARG, "null argument");
+    const bool xz = (kind & DRP_FILE_LAYOUT_XZ) != 0;
+    kind &= ~DRP_FILE_LAYOUT_XZ;
     if (kind != DRP_FILE_FIELDS && kind != DRP_FILE_FG) return drp_fail(DRP_ERR_ARG, "unknown file kind %d", kind);
     const drp_config& c = L->cfg;
-    int rc = check_split(c, ranks_x, ranks_y);
+    int rc = check_split(c, ranks_x, ranks_y, xz);
     if (rc) return rc;
     const size_t N = (size_t)L->n;
     std::vector<double> a, b;
@@ -87,13 +99,13 @@ extern "C" int drp_write_reference_files(drp_lattice* L, const char* dir, int64_
     }
     if (rc) return rc;
     for (int r = 0; r < ranks_x * ranks_y; r++) {
-        const Block B = block_of(c, r, ranks_x, ranks_y);
+        const Block B = block_of(c, r, ranks_x, ranks_y, xz);
         const std::string name = file_name(dir, r, n, kind == DRP_FILE_FG);
         RecordFile F;
         F.nval = kind == DRP_FILE_FIELDS ? 5 : 2;
         F.f = std::fopen(name.c_str(), "wb");
         if (!F.f) return drp_fail(DRP_ERR_ARG, "cannot write %s: %s", name.c_str(), std::strerror(errno));
-        for (int z = 0; z < c.nz; z++)
+        for (int z = B.z0; z < B.z0 + B.zp; z++)
             for (int y = B.y0; y < B.y0 + B.yp; y++)
                 for (int x = B.x0; x < B.x0 + B.xp; x++) {
                     const size_t s = (size_t)x + (size_t)c.nx * ((size_t)y + (size_t)c.ny * z);
@@ -117,22 +129,25 @@ extern "C" int drp_write_reference_files(drp_lattice* L, const char* dir, int64_
 extern "C" int drp_read_reference_files(drp_lattice* L, const char* dir, int64_t n, int ranks_x, int ranks_y,
                                         int kind) {
     if (!L || !dir) return drp_fail(DRP_ERR_ARG, "null argument");
-    if (kind != DRP_FILE_FG && kind != DRP_FILE_G) return drp_fail(DRP_ERR_ARG, "unknown file kind %d", kind);
+    const bool xz = (kind & DRP_FILE_LAYOUT_XZ) != 0;
+    kind &= ~DRP_FILE_LAYOUT_XZ;
+    if (kind != DRP_FILE_FG && kind != DRP_FILE_G && kind != DRP_FILE_FG_G)
+        return drp_fail(DRP_ERR_ARG, "unknown file kind %d", kind);
     if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
     if (n < 0) return drp_fail(DRP_ERR_ARG, "file number %lld < 0", (long long)n);
     const drp_config& c = L->cfg;
-    int rc = check_split(c, ranks_x, ranks_y);
+    int rc = check_split(c, ranks_x, ranks_y, xz);
     if (rc) return rc;
     const size_t N = (size_t)L->n;
     std::vector<double> f(kind == DRP_FILE_FG ? 15 * N : 0), g(15 * N);
     for (int r = 0; r < ranks_x * ranks_y; r++) {
-        const Block B = block_of(c, r, ranks_x, ranks_y);
+        const Block B = block_of(c, r, ranks_x, ranks_y, xz);
         const std::string name = file_name(dir, r, n, true);
         RecordFile F;
-        F.nval = kind == DRP_FILE_FG ? 2 : 1;
+        F.nval = kind == DRP_FILE_G ? 1 : 2;
         F.f = std::fopen(name.c_str(), "rb");
         if (!F.f) return drp_fail(DRP_ERR_ARG, "cannot read %s: %s", name.c_str(), std::strerror(errno));
-        for (int z = 0; z < c.nz; z++)
+        for (int z = B.z0; z < B.z0 + B.zp; z++)
             for (int y = B.y0; y < B.y0 + B.yp; y++)
                 for (int x = B.x0; x < B.x0 + B.xp; x++) {
                     const size_t s = (size_t)x + (size_t)c.nx * ((size_t)y + (size_t)c.ny * z);
@@ -143,7 +158,7 @@ extern "C" int drp_read_reference_files(drp_lattice* L, const char* dir, int64_t
                             f[q * N + s] = v[0];
                             g[q * N + s] = v[1];
                         } else {
-                            g[q * N + s] = v[0];
+                            g[q * N + s] = kind == DRP_FILE_FG_G ? v[1] : v[0];   // FG_G: f_i is read and dropped
                         }
                     }
                 }

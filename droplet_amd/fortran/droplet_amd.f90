@@ -19,7 +19,8 @@ module droplet_amd
     integer(c_int), parameter :: DRP_WALL_NONE = 0, DRP_WALL_Y = 2, DRP_WALL_Z = 3
     integer(c_int), parameter :: DRP_FORCE_NONE = 0, DRP_FORCE_TAYLOR_GREEN = 1, DRP_FORCE_FIELD = 2, &
                                  DRP_FORCE_SPECTRAL = 3
-    integer(c_int), parameter :: DRP_FILE_FIELDS = 0, DRP_FILE_FG = 1, DRP_FILE_G = 2
+    integer(c_int), parameter :: DRP_FILE_FIELDS = 0, DRP_FILE_FG = 1, DRP_FILE_G = 2, DRP_FILE_FG_G = 3, &
+                                 DRP_FILE_LAYOUT_XZ = 256
     integer(c_int), parameter :: DRP_GRAD_MOBILITY = 0, DRP_GRAD_KAPPA_G = 1, DRP_GRAD_WALL_U = 2
     integer(c_int), parameter :: DRP_FIELD_PHI = 0, DRP_FIELD_P = 1, DRP_FIELD_U1 = 2, DRP_FIELD_U2 = 3, &
                                  DRP_FIELD_U3 = 4

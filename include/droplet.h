@@ -252,12 +252,20 @@ void drp_tape_destroy(drp_tape* t);
  *   DRP_FILE_FIELDS <dir>/0_<r>_<n>.bin     records (phi, u1, u2, u3, p)    = output (:695-719)
  *   DRP_FILE_FG     <dir>/0_<r>_<n>_fg.bin  records (f_i, g_i)               = outputfg (:721-743)
  *   DRP_FILE_G      <dir>/0_<r>_<n>_fg.bin  records g_i (momentum only)      = step_input input (:789-828)
+ *   DRP_FILE_FG_G   <dir>/0_<r>_<n>_fg.bin  records (f_i, g_i), g_i used     = step_input input of
+ *                   program/LES_drop.f90 (:919-945), which reads outputfg files and keeps g
  * drp_read_reference_files with DRP_FILE_FG is a restart: populations, moments,
  * the step counter (= n) and the phi range of the restored state; with
- * DRP_FILE_G it injects the momentum populations like drp_inject_momentum. */
+ * DRP_FILE_G or DRP_FILE_FG_G it injects the momentum populations like
+ * drp_inject_momentum (read only).
+ * DRP_FILE_LAYOUT_XZ, OR-ed into kind, selects program/LES_drop.f90's
+ * decomposition (:153-176): ranks_x slabs in x times ranks_y slabs in z, y whole,
+ * rank r = (z slab) * ranks_x + (x slab). */
 #define DRP_FILE_FIELDS 0
 #define DRP_FILE_FG 1
 #define DRP_FILE_G 2
+#define DRP_FILE_FG_G 3
+#define DRP_FILE_LAYOUT_XZ 0x100
 int drp_write_reference_files(drp_lattice* lat, const char* dir, int64_t n, int ranks_x, int ranks_y, int kind);
 int drp_read_reference_files(drp_lattice* lat, const char* dir, int64_t n, int ranks_x, int ranks_y, int kind);
 

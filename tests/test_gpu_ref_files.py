@@ -6,7 +6,10 @@ uses on the reference programs' own files (tests/golden/fortran_io.py):
 - `outputfg` files round-trip, and a restart from them continues bit-identically
   (step counter and phi range restored);
 - the golden injection case run from g-input files written like the reference's
-  input reproduces the reference's fields bit for bit.
+  input reproduces the reference's fields bit for bit;
+- program/LES_drop.f90's x-z decomposition: output files assemble like the
+  reference's, and the LES golden case run from (f, g) input files in that
+  layout (g kept, as LES_drop.f90 reads them) reproduces its fields bit for bit.
 """
 import os
 import sys
@@ -14,7 +17,8 @@ import sys
 import numpy as np
 import pytest
 
-from droplet_amd.abi import DRP_FILE_FG, DRP_FILE_FIELDS, DRP_FILE_G, DRP_WALL_Y, POP_MOM, POP_PHI, make_config
+from droplet_amd.abi import (DRP_FILE_FG, DRP_FILE_FG_G, DRP_FILE_FIELDS, DRP_FILE_G, DRP_FILE_LAYOUT_XZ, DRP_WALL_Y,
+                             POP_MOM, POP_PHI, make_config)
 from tests import golden_data as gd
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
@@ -113,3 +117,37 @@ def test_file_errors(tmp_path):
         lat.read_reference_files(str(tmp_path), 5, DRP_FILE_FG, 4, 1)       # wrong split: sizes differ
     with pytest.raises(dl.DropletError):
         lat.read_reference_files(str(tmp_path), 5, DRP_FILE_G, 8, 1)        # (f, g) records read as g-only
+
+
+def test_xz_layout_output_files(tmp_path):
+    dl = _dl()
+    N = 16
+    lat = dl.Lattice(make_config(N, kappa_g=0.01))
+    lat.init_droplets([(7.5, 7.5, 7.5, 8.0)])
+    lat.step(5)
+    f = lat.fields()
+    for rx, rz in ((8, 2), (4, 4)):
+        d = tmp_path / f"xz{rx}{rz}"
+        d.mkdir()
+        lat.write_reference_files(str(d), 5, DRP_FILE_FIELDS | DRP_FILE_LAYOUT_XZ, rx, rz)
+        got = fio.assemble_xz(str(d), 5, N, rx, rz)
+        for k, name in enumerate(("phi", "p", "u1", "u2", "u3")):
+            assert np.array_equal(got[k], f[name]), (rx, rz, name)
+
+
+def test_les_golden_from_reference_fg_input(tmp_path):
+    dl = _dl()
+    z, meta = gd.load("les16")
+    g = z["g_injected"]
+    fio.write_fg_input(str(tmp_path), 7, np.full_like(g, 12345.0), g, 8, 2)   # f values are read and dropped
+    lat = dl.Lattice(gd.config_for(meta))
+    lat.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    lat.step(meta["step_input"] - 1)
+    lat.read_reference_files(str(tmp_path), 7, DRP_FILE_FG_G | DRP_FILE_LAYOUT_XZ, 8, 2)
+    n = meta["steps"][0]
+    lat.step(n - lat.step_count)
+    got = lat.fields()
+    for k, name in enumerate(("phi", "p", "u1", "u2", "u3")):
+        assert np.array_equal(got[name], z[f"fields_{n}"][k]), name
+    with pytest.raises(dl.DropletError):   # a 4 x 2 split needs 512 nodes per file, these hold 256
+        lat.read_reference_files(str(tmp_path), 7, DRP_FILE_FG_G | DRP_FILE_LAYOUT_XZ, 4, 2)
