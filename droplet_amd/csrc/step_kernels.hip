@@ -189,7 +189,10 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const double rmax = from_ordered_bits(a.range[2 * a.slot_read + 0]);
     const double rmin = from_ordered_bits(a.range[2 * a.slot_read + 1]);
     const double range = rmax - rmin;
-    for (int h = tid; h < H1; h += NT) {
+    static_for<0, (H1 + NT - 1) / NT>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const int h = tid + k * NT;
+        if (h >= H1) return;
         const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
         const int c = (hx + 1) + HX2 * ((hy + 1) + HY2 * (hz + 1));
         const double ph = s_phi[c];
@@ -221,7 +224,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             const double nu = frac * a.dnu + a.nu1;
             s_tau[h] = 0.5 + 3.0 * nu;
         }
-    }
+    });
     __syncthreads();
 
     // ---------------- phase 3: collide-stream of this thread's population set
