@@ -9,12 +9,12 @@ from droplet_amd.abi import DRP_FP32_SHIFTED, DRP_FP64, DRP_WALL_Y, kappa_g_from
 pytestmark = pytest.mark.gpu
 
 
-def _members(precision, n=5, seed=4):
+def _members(precision, n=5, seed=4, nu2=0.1):
     rng = np.random.default_rng(seed)
     cfgs, drops = [], []
     for i in range(n):
         N = int(rng.choice([24, 32, 40, 48]))
-        cfgs.append(make_config(N, precision=precision, nu1=0.1, nu2=0.1, kappa_g=kappa_g_from_sigma(0.01),
+        cfgs.append(make_config(N, precision=precision, nu1=0.1, nu2=nu2, kappa_g=kappa_g_from_sigma(0.01),
                                 wall_axis=DRP_WALL_Y, wall_u=float(rng.uniform(0.005, 0.03)),
                                 wall_start=int(rng.integers(1, 9))))
         off = rng.uniform(-2, 2, 3)
@@ -24,9 +24,10 @@ def _members(precision, n=5, seed=4):
 
 @pytest.mark.parametrize("precision", [DRP_FP64, DRP_FP32_SHIFTED])
 @pytest.mark.parametrize("graph", [True, False])
-def test_batch_equals_individual_runs(precision, graph):
+@pytest.mark.parametrize("nu2", [0.1, 0.2], ids=["nu_const", "nu_phi"])
+def test_batch_equals_individual_runs(precision, graph, nu2):
     from droplet_amd.lattice import Batch, Lattice
-    cfgs, drops = _members(precision)
+    cfgs, drops = _members(precision, nu2=nu2)
     b = Batch(cfgs)
     b.use_graph(graph)
     for m, d in zip(b.members, drops):
