@@ -67,10 +67,15 @@ def test_output_files_layout(tmp_path):
         assert np.array_equal(got2[k], f2[name])
 
 
-def test_outputfg_restart_is_bit_identical(tmp_path):
+@pytest.mark.parametrize("les", [False, True], ids=["nu_phi", "les"])
+def test_outputfg_restart_is_bit_identical(tmp_path, les):
+    """A restart continues bit for bit: populations, step counter and phi range
+    are restored; with LES (all neighbours current) the restored state's tau
+    equals the one the continuing run computed at the end of its step."""
     dl = _dl()
     N = 16
-    cfg = make_config(N, kappa_g=0.01, nu2=0.2, wall_axis=DRP_WALL_Y, wall_u=0.01)   # nu(phi): the range matters
+    kw = dict(les_cs=0.15, les_delta=2.0) if les else {}
+    cfg = make_config(N, kappa_g=0.01, nu2=0.2, wall_axis=DRP_WALL_Y, wall_u=0.01, **kw)   # nu(phi): the range matters
     a = dl.Lattice(cfg)
     a.init_droplets([(7.5, 7.5, 7.5, 8.0)])
     a.step(20)
