@@ -73,6 +73,7 @@ class Stats(C.Structure):
         ("phi_mass", C.c_double), ("phi_min", C.c_double), ("phi_max", C.c_double),
         ("kinetic", C.c_double), ("dissipation", C.c_double), ("interface_area", C.c_double),
         ("free_energy", C.c_double), ("p_mean", C.c_double), ("momentum", C.c_double * 3),
+        ("interaction", C.c_double),
     ]
 
     def as_dict(self) -> dict:

@@ -207,7 +207,7 @@ __global__ __launch_bounds__(LES_TX* LES_TY* LES_TZ) void les_tau_fp64(Grid G, P
 
 // Diagnostics: per-workgroup partial sums, reduced in fixed order on the host.
 // [0] sum phi  [1] sum 0.5|u|^2  [2] sum (d_a u_b + d_b u_a)^2  [3] sum |grad phi|
-// [4] sum psi(phi)  [5] sum p  [6..8] sum u  [9] unused
+// [4] sum psi(phi)  [5] sum p  [6..8] sum u  [9] sum_ab d_a phi d_b phi S_ab
 __global__ void reduce_fp64(Grid G, Phys P, const double* __restrict__ mac, double* __restrict__ partial) {
     __shared__ double sh[NPART][4];
     const int64_t N = (int64_t)G.nx * G.ny * G.nz;
@@ -235,6 +235,13 @@ __global__ void reduce_fp64(Grid G, Phys P, const double* __restrict__ mac, doub
         v[6] = u1;
         v[7] = u2;
         v[8] = u3;
+        // interface-strain interaction, ref drop_analysis.f90 korteweg_cal /
+        // strain_cal / interaction_cal (:830-882): beta outer, alpha inner
+        const double gp[3] = {g0, g1, g2};
+        double it = 0.0;
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) it = it + (gp[a] * gp[b]) * (0.5 * (gu[a][b] + gu[b][a]));
+        v[9] = it;
     }
     for (int k = 0; k < NPART; k++) {
         double t = v[k];

@@ -563,6 +563,7 @@ extern "C" int drp_reduce(drp_lattice* L, drp_stats* st) {
     st->momentum[0] = s[6];
     st->momentum[1] = s[7];
     st->momentum[2] = s[8];
+    st->interaction = s[9];
     return DRP_OK;
 }
 

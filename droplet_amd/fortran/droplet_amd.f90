@@ -44,6 +44,7 @@ module droplet_amd
     type, bind(C) :: drp_stats
         real(c_double) :: phi_mass, phi_min, phi_max, kinetic, dissipation, interface_area, free_energy, p_mean
         real(c_double) :: momentum(3)
+        real(c_double) :: interaction
     end type drp_stats
 
     interface

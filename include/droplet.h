@@ -116,6 +116,8 @@ typedef struct drp_stats {
     double free_energy;   /* sum phi T ln(phi/(1-b phi)) - a phi^2 (bulk part)         */
     double p_mean;        /* <p>                                                       */
     double momentum[3];   /* sum u                                                     */
+    double interaction;   /* sum over nodes of sum_ab d_a phi d_b phi S_ab, S = (grad u + grad u^T)/2:
+                             the interface-strain interaction of ref drop_analysis.f90:797-882 */
 } drp_stats;
 
 typedef struct drp_lattice drp_lattice; /* opaque; owns all device memory */

@@ -608,7 +608,7 @@ int ora_reduce(ora_lattice* L, drp_stats* st) {
         for (int y = 0; y < c->ny; y++)
             for (int x = 0; x < c->nx; x++) stencils_at(L, x, y, z);
     memset(st, 0, sizeof(*st));
-    double mass = 0, ke = 0, eps = 0, area = 0, fe = 0, pm = 0, m0 = 0, m1 = 0, m2 = 0;
+    double mass = 0, ke = 0, eps = 0, area = 0, fe = 0, pm = 0, m0 = 0, m1 = 0, m2 = 0, inter = 0;
     for (int64_t s = 0; s < L->n; s++) {
         double phi = L->phi[s];
         mass += phi;
@@ -622,6 +622,17 @@ int ora_reduce(ora_lattice* L, drp_stats* st) {
         eps += e;
         double g0 = L->gp[s], g1 = L->gp[L->n + s], g2 = L->gp[2 * L->n + s];
         area += sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        /* interaction_cal, program/drop_analysis.f90:864-882: sum over beta, alpha of
+           korteweg (d_alpha phi d_beta phi, :848-862) times strain (:830-846) */
+        {
+            const double gp[3] = {g0, g1, g2};
+            double it = 0.0;
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++)
+                    it = it + (gp[a] * gp[b]) *
+                                  (0.5 * (L->gu[(a * 3 + b) * L->n + s] + L->gu[(b * 3 + a) * L->n + s]));
+            inter += it;
+        }
         fe += phi * c->T * log(phi / (1.0 - c->b * phi)) - c->a * phi * phi;
         pm += L->p[s];
         m0 += L->u1[s];
@@ -640,6 +651,7 @@ int ora_reduce(ora_lattice* L, drp_stats* st) {
     st->momentum[0] = m0;
     st->momentum[1] = m1;
     st->momentum[2] = m2;
+    st->interaction = inter;
     return 0;
 }
 

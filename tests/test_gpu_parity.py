@@ -235,7 +235,7 @@ def test_reduce_vs_oracle():
         o.step(30)
     a, b = lat.reduce(), ora.reduce()
     for k in ("phi_mass", "phi_min", "phi_max", "kinetic", "dissipation", "interface_area", "free_energy",
-              "p_mean"):
+              "p_mean", "interaction"):
         assert abs(a[k] - b[k]) <= 1e-11 * max(1.0, abs(b[k])), (k, a[k], b[k])
     np.testing.assert_allclose(a["momentum"], b["momentum"], rtol=1e-9, atol=1e-12)
 

@@ -202,6 +202,27 @@ def test_diagnostics_closed_forms():
     assert math.isclose(st["free_energy"], N ** 3 * psi, rel_tol=1e-12)
     assert st["interface_area"] < 1e-12
     assert math.isclose(st["phi_mass"], N ** 3 * phi0, rel_tol=1e-14)
+    assert abs(st["interaction"]) < 1e-15
+
+
+def test_interaction_closed_form():
+    """Interface-strain interaction sum_ab d_a phi d_b phi S_ab for phi = phi0 + A sin(kx),
+    u1 = B sin(2kx): the D3Q15 gradient of a function of x is the central difference,
+    so d_x phi = A sin(k) cos(kx), S_xx = B sin(2k) cos(2kx), and the sum over the grid
+    is A^2 B N^3 sin(k)^2 sin(2k) / 4 (the mean of cos^2 t cos 2t is 1/4).  A missing
+    1/2 in S, a missing square of grad phi or the gradient of another field would miss it."""
+    N, A, B = 16, 0.3, 0.01
+    cfg = make_config(N)
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    k = 2 * np.pi / N
+    o = OracleLattice(cfg)
+    o.init_fields(3.0 + A * np.sin(k * x), 1 / 3, B * np.sin(2 * k * x), 0, 0)
+    want = A * A * B * N ** 3 * math.sin(k) ** 2 * math.sin(2 * k) / 4
+    assert math.isclose(o.reduce()["interaction"], want, rel_tol=1e-12)
+    # u1 = B sin(k y) has only S_xy = S_yx; with phi varying in x only (d_y phi = 0)
+    # every product vanishes
+    o.init_fields(3.0 + A * np.sin(k * x), 1 / 3, B * np.sin(k * y), 0, 0)
+    assert abs(o.reduce()["interaction"]) < 1e-15
 
 
 def test_threads_do_not_change_results():
