@@ -293,7 +293,7 @@ int drp_copy_bandwidth_soa(size_t bytes_per_stream, int streams, int reps, doubl
    volumes[k] (nodes) and first[k] (smallest node index x + nx (y + ny z) in the
    cluster) for k < min(count, cap), ordered by first; labels (nullable,
    n = nodes): per node the first index of its cluster, -1 outside.  The
-   reference's scan links 5 of the 12 edge neighbours as well and follows
+   reference's scan also links some diagonal (edge) neighbours and follows
    periodic wraps only partly; on clusters that neither touch diagonally nor
    cross a wrap the counts agree.  Needs fewer than 2^30 nodes. */
 int drp_droplet_census(drp_lattice* lat, double threshold, int64_t* count, int64_t* volumes, int64_t* first,
