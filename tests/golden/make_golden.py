@@ -165,9 +165,9 @@ def injected_momentum(N, seed, amp=0.02, noise=0.002):
     return g
 
 
-def case_les(name, N, steps, step_input, seed, delta, Cs, Re, Dv, D, sigma, umax=0.1):
-    """Smagorinsky LES with a droplet, program/LES_drop.f90 on 16 ranks (8 slabs in x
-    times 2 in z; its z-neighbour set-up needs at least 2 z slabs, :184-192): momentum populations injected at step_input from (f, g) pair files (the
+def case_les(name, N, steps, step_input, seed, delta, Cs, Re, Dv, D, sigma, umax=0.1, ranks_z=2):
+    """Smagorinsky LES with a droplet, program/LES_drop.f90 on 8 * ranks_z ranks (8 slabs
+    in x times ranks_z in z; its z-neighbour set-up needs at least 2 z slabs, :184-192): momentum populations injected at step_input from (f, g) pair files (the
     f values are ignored, :921-945), Taylor-Green force from then on (:774-798),
     eddy viscosity from grad u at the end of each step with halos exchanged at its
     start (physics :598-693).  Fields written when n is a multiple of 500."""
@@ -181,17 +181,17 @@ def case_les(name, N, steps, step_input, seed, delta, Cs, Re, Dv, D, sigma, umax
     f_dummy = np.full_like(g, 12345.0)
     with tempfile.TemporaryDirectory() as tmp:
         os.makedirs(os.path.join(tmp, "in"))
-        write_fg_input(os.path.join(tmp, "in"), 7, f_dummy, g, 8, 2)
-        run(exe, 16, tmp)
-        fields = assemble_xz(os.path.join(tmp, "out"), steps, N, 8, 2)
+        write_fg_input(os.path.join(tmp, "in"), 7, f_dummy, g, 8, ranks_z)
+        run(exe, 8 * ranks_z, tmp)
+        fields = assemble_xz(os.path.join(tmp, "out"), steps, N, 8, ranks_z)
     pi = math.acos(-1.0)
     nu1 = umax * Dv / Re
     data = {f"fields_{steps}": fields, "g_injected": g}
-    meta = dict(case=name, program="program/LES_drop.f90", nranks=16, params=params, step_input=step_input,
+    meta = dict(case=name, program="program/LES_drop.f90", nranks=8 * ranks_z, params=params, step_input=step_input,
                 phys=dict(nx=N, ny=N, nz=N, nu1=nu1, nu2=1.0 * nu1, tau_f=0.7, kappa_f=0.01,
                           kappa_g=(sigma / 1.7039) ** (1.0 / 0.9991), drop=[0.5 * (N - 1)] * 3 + [D], p_init=1.0 / 3.0,
                           force_amp=2.0 * pi * pi * umax * umax / (Dv * Re), force_k=pi / Dv,
-                          les_cs=Cs, les_delta=delta * 1.0, les_ranks_x=8, les_ranks_y=1, les_ranks_z=2),
+                          les_cs=Cs, les_delta=delta * 1.0, les_ranks_x=8, les_ranks_y=1, les_ranks_z=ranks_z),
                 steps=[steps],
                 sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "program/LES_drop.f90")))
     return data, meta
@@ -272,6 +272,9 @@ CASES = {
     "dsd32": lambda: case_dsd("dsd32", 32, seed=31),
     "les16": lambda: case_les("les16", 16, 500, step_input=3, seed=21, delta=4.0, Cs=0.173, Re=80.0, Dv=8.0,
                               D=8.0, sigma=1e-4),
+    # the same on 32 ranks: 8 x-slabs times 4 z-slabs of 4 nodes
+    "les16z4": lambda: case_les("les16z4", 16, 500, step_input=3, seed=22, delta=4.0, Cs=0.173, Re=80.0, Dv=8.0,
+                                D=8.0, sigma=1e-4, ranks_z=4),
 }
 
 

@@ -130,18 +130,19 @@ def test_les_vs_oracle():
     _compare(lat.fields(), ora.fields(), "LES")
 
 
-def test_les_golden():
-    """program/LES_drop.f90 on 8 x 1 x 2 ranks: eddy viscosity at the end of each
-    step with the values across rank blocks one step old, momentum injection at
-    step 3 and the Taylor-Green force, 500 steps at 16^3."""
-    z, meta = gd.load("les16")
+@pytest.mark.parametrize("name", ["les16", "les16z4"])
+def test_les_golden(name):
+    """program/LES_drop.f90 on 8 x 1 x 2 and 8 x 1 x 4 ranks: eddy viscosity at the
+    end of each step with the values across rank blocks one step old, momentum
+    injection at step 3 and the Taylor-Green force, 500 steps at 16^3."""
+    z, meta = gd.load(name)
     lat = Lattice(gd.config_for(meta))
     lat.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
     lat.step(meta["step_input"] - 1)
     lat.inject_momentum(z["g_injected"])
     for n in meta["steps"]:
         lat.step(n - lat.step_count)
-        assert _compare(lat.fields(), z[f"fields_{n}"], f"les16@{n}")
+        assert _compare(lat.fields(), z[f"fields_{n}"], f"{name}@{n}")
 
 
 def test_les_rank_blocks_vs_oracle():

@@ -70,12 +70,13 @@ def test_moving_walls_bit_exact():
     np.testing.assert_array_equal(got["p"], z[f"p_{n}"])
 
 
-def test_les_injection_bit_exact():
+@pytest.mark.parametrize("name", ["les16", "les16z4"])
+def test_les_injection_bit_exact(name):
     """program/LES_drop.f90: Smagorinsky eddy viscosity from grad u at the end of each
-    step, with the values across its 8 x 2 slab boundaries and periodic wraps one
-    step old (les_ranks 8 x 1 x 2), after a momentum injection and with
-    the Taylor-Green force; 500 steps at 16^3."""
-    z, meta = gd.load("les16")
+    step, with the values across its slab boundaries (8 x-slabs times 2 or 4 z-slabs)
+    and periodic wraps one step old, after a momentum injection and with the
+    Taylor-Green force; 500 steps at 16^3."""
+    z, meta = gd.load(name)
     o = OracleLattice(gd.config_for(meta))
     o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
     o.step(meta["step_input"] - 1)
@@ -85,14 +86,15 @@ def test_les_injection_bit_exact():
         ref = z[f"fields_{n}"]
         got = o.fields()
         for k, fld in enumerate(FIELDS):
-            np.testing.assert_array_equal(got[fld], ref[k], err_msg=f"les16 step {n} {fld}")
+            np.testing.assert_array_equal(got[fld], ref[k], err_msg=f"{name} step {n} {fld}")
 
 
 def test_les_fixture_pins_the_decomposition():
-    """The same run with every neighbour current, or with the z split left out,
-    misses the fixture: it pins the one-step-old values across slab boundaries."""
+    """The same run with every neighbour current, or with another z split, misses
+    the fixtures: they pin the one-step-old values across slab boundaries."""
     z, meta = gd.load("les16")
-    for over in (dict(les_ranks_x=0, les_ranks_y=0, les_ranks_z=0), dict(les_ranks_x=8, les_ranks_y=1, les_ranks_z=1)):
+    for over in (dict(les_ranks_x=0, les_ranks_y=0, les_ranks_z=0), dict(les_ranks_x=8, les_ranks_y=1, les_ranks_z=1),
+                 dict(les_ranks_x=8, les_ranks_y=1, les_ranks_z=4)):
         o = OracleLattice(gd.config_for(meta, **over))
         o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
         o.step(meta["step_input"] - 1)
