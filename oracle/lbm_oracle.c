@@ -20,7 +20,8 @@
  * ORACLE_PLANT (compile-time, default 0) plants one known mistake so the tests
  * can show each pin catches it: 1 drops the Korteweg term of geq, 2 flips the
  * sign of the body force, 3 divides the Laplacian by 10 instead of 5, 4 moves
- * the bottom wall with +U instead of -U.  Only tests build planted variants.
+ * the bottom wall with +U instead of -U, 5 drops the square root of the
+ * interface area's |grad phi|.  Only tests build planted variants.
  *
  * Parity pins (tests/test_oracle_*.py): golden vectors produced by the compiled
  * reference (tests/golden/), lattice identities, moment identities, polynomial
@@ -621,7 +622,7 @@ int ora_reduce(ora_lattice* L, drp_stats* st) {
             }
         eps += e;
         double g0 = L->gp[s], g1 = L->gp[L->n + s], g2 = L->gp[2 * L->n + s];
-        area += sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        area += (ORACLE_PLANT == 5 ? (g0 * g0 + g1 * g1 + g2 * g2) : sqrt(g0 * g0 + g1 * g1 + g2 * g2));
         /* interaction_cal, program/drop_analysis.f90:864-882: sum over beta, alpha of
            korteweg (d_alpha phi d_beta phi, :848-862) times strain (:830-846) */
         {

@@ -225,6 +225,53 @@ def test_interaction_closed_form():
     assert abs(o.reduce()["interaction"]) < 1e-15
 
 
+def _area_closed_form(libpath=None):
+    """Interface area sum |grad phi| (At_cal, program/drop_analysis.f90:445-474) of
+    phi = phi0 + A sin(kx) + B sin(ky).  Five D3Q15 directions have c_x = +1 and
+    five c_x = -1, so the gradient of a function of x alone is the central
+    difference (phi(x+1) - phi(x-1)) / 2: d_x phi = A sin(k) cos(kx), d_y phi =
+    B sin(k) cos(ky), and the area is N sin(k) sum_xy sqrt(A^2 cos^2 kx + B^2 cos^2 ky).
+    A missing square root, a wrong gradient scale or a component left out misses it."""
+    N, A, B = 16, 0.3, 0.2
+    k = 2 * np.pi / N
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    o = OracleLattice(make_config(N), libpath=libpath)
+    o.init_fields(3.0 + A * np.sin(k * x) + B * np.sin(k * y), 1 / 3, 0, 0, 0)
+    xs = np.arange(N)
+    cx, cy = np.meshgrid(np.cos(k * xs), np.cos(k * xs), indexing="ij")
+    want = N * math.sin(k) * float(np.sqrt(A * A * cx * cx + B * B * cy * cy).sum())
+    return math.isclose(o.reduce()["interface_area"], want, rel_tol=1e-12)
+
+
+def test_interface_area_closed_form():
+    assert _area_closed_form()
+    # one-dimensional profile: N^2 A sin(k) sum_x |cos kx|
+    N, A = 16, 0.3
+    k = 2 * np.pi / N
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    o = OracleLattice(make_config(N))
+    o.init_fields(3.0 + A * np.sin(k * z), 1 / 3, 0, 0, 0)
+    want = N * N * A * math.sin(k) * float(np.abs(np.cos(k * np.arange(N))).sum())
+    assert math.isclose(o.reduce()["interface_area"], want, rel_tol=1e-12)
+
+
+def test_mean_pressure_and_momentum_closed_forms():
+    """<p> and sum u (reduction rows of drp_stats): p = p0 + P sin(kz) cos(kx) has
+    mean p0; u1 = U sin(ky) sums to 0, a constant u2 = W to N^3 W and
+    u3 = V cos(kx) + 2W to 2 N^3 W, so a mean taken as a sum, a component in the
+    wrong slot or a sum over a partial grid misses them."""
+    N, P, U, V, W, p0 = 16, 0.01, 0.02, 0.015, 0.003, 0.3
+    k = 2 * np.pi / N
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    o = OracleLattice(make_config(N))
+    o.init_fields(3.0, p0 + P * np.sin(k * z) * np.cos(k * x), U * np.sin(k * y), W, V * np.cos(k * x) + 2 * W)
+    st = o.reduce()
+    assert math.isclose(st["p_mean"], p0, rel_tol=1e-13)
+    assert abs(st["momentum"][0]) < 1e-14
+    assert math.isclose(st["momentum"][1], N ** 3 * W, rel_tol=1e-13)
+    assert math.isclose(st["momentum"][2], 2 * N ** 3 * W, rel_tol=1e-13)
+
+
 def test_threads_do_not_change_results():
     cfg = make_config(16, kappa_g=0.01, nu2=0.25, wall_axis=DRP_WALL_Y, wall_u=0.02)
     outs = []
@@ -293,6 +340,7 @@ def _force_golden_matches(libpath):
     (2, [_force_golden_matches]),                     # body-force sign flipped
     (3, [_golden_periodic_matches, _stencil_exact]),  # Laplacian scaled wrongly
     (4, [_couette_holds]),                            # bottom wall velocity sign
+    (5, [_area_closed_form]),                         # interface area without the square root
 ])
 def test_planted_mistakes_are_caught(tmp_path, plant, checks):
     """Every planted mistake makes its pins fail, while the clean build passes them."""
