@@ -21,7 +21,7 @@ LIB = os.path.join(OUT_DIR, "libdroplet_amd.so")
 ARCH = os.environ.get("DROPLET_ARCH", "gfx950")
 
 SOURCES = ["step_kernels.hip", "step_fp32.hip", "aux_kernels.hip", "aux_fp32.hip", "adjoint.hip", "spectral.hip", "probe.hip",
-           "census.hip",
+           "census.hip", "spectral_diag.hip",
            "lattice.cpp", "deformation.cpp", "fp32_shifted.cpp", "batch.cpp", "adjoint.cpp", "ref_files.cpp"]
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function", "-Wno-pass-failed",
@@ -35,15 +35,31 @@ def hipcc() -> str:
     raise FileNotFoundError("hipcc not found")
 
 
-def build(verbose: bool = False, jobs: int = 4) -> str:
+# the step kernels are compiled once per tile shape (-DDRP_STEP_SHAPE=k) plus once
+# for their dispatcher, so that the template instances build in parallel
+PER_SHAPE = ("step_kernels.hip", "step_fp32.hip")
+SHAPES = 7
+
+
+def units():
+    """(source, extra flags, object name) of every translation unit."""
+    out = []
+    for src in SOURCES:
+        out.append((src, [], src + ".o"))
+        if src in PER_SHAPE:
+            out += [(src, [f"-DDRP_STEP_SHAPE={k}"], f"{src}.{k}.o") for k in range(SHAPES)]
+    return out
+
+
+def build(verbose: bool = False, jobs: int = 8) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     cc = hipcc()
     with tempfile.TemporaryDirectory(prefix="droplet_build_") as tmp:
         objs, procs = [], []
-        for src in SOURCES:
+        for src, extra, oname in units():
             path = os.path.join(CSRC, src)
-            obj = os.path.join(tmp, src + ".o")
-            cmd = [cc, f"--offload-arch={ARCH}", "-x", "hip", *COMMON, *DEVICE_FLAGS,
+            obj = os.path.join(tmp, oname)
+            cmd = [cc, f"--offload-arch={ARCH}", "-x", "hip", *COMMON, *DEVICE_FLAGS, *extra,
                    f"-I{CSRC}", "-c", path, "-o", obj]
             if verbose:
                 print(" ".join(cmd), file=sys.stderr)

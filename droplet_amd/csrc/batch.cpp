@@ -194,6 +194,9 @@ extern "C" int drp_batch_create(const drp_config* cfgs, int count, drp_batch** o
     }
     if (cfgs[0].les_cs != 0.0 && cfgs[0].les_delta != 0.0)
         return drp_fail(DRP_ERR_UNSUPPORTED, "LES members are not supported in a batch");
+    for (int i = 0; i < count; i++)
+        if (cfgs[i].model != DRP_MODEL_FREE_ENERGY)
+            return drp_fail(DRP_ERR_UNSUPPORTED, "batch members must use the free-energy model");
     if (cfgs[0].force_kind == DRP_FORCE_SPECTRAL)
         return drp_fail(DRP_ERR_UNSUPPORTED, "spectral forcing is not supported in a batch");
     drp_batch* B = new (std::nothrow) drp_batch();
@@ -288,6 +291,37 @@ extern "C" int drp_batch_step(drp_batch* B, int nsteps) {
         }
     }
     return DRP_OK;
+}
+
+// Time every tile shape on a temporary batch of the same members (each with a
+// centred droplet of diameter 0.4 nx), nsteps steps per shape after 2 warm-up
+// steps, and return the fastest shape.
+extern "C" int drp_batch_autotune(const drp_config* cfgs, int count, int nsteps, int* best, double* ms_out) {
+    if (!cfgs || count < 1 || !best || nsteps < 1) return drp_fail(DRP_ERR_ARG, "bad batch autotune arguments");
+    drp_batch* B = nullptr;
+    int rc = drp_batch_create(cfgs, count, &B);
+    if (rc) return rc;
+    for (int i = 0; i < count && !rc; i++) {
+        const drp_config& c = cfgs[i];
+        const drp_drop d{0.5 * (c.nx - 1), 0.5 * (c.ny - 1), 0.5 * (c.nz - 1), 0.4 * c.nx};
+        rc = drp_init_droplets(B->members[i], &d, 1, 1.0 / 3.0);
+    }
+    double bestms = 1e300;
+    for (int s = 0; s < SHAPE_COUNT && !rc; s++) {
+        rc = drp_batch_set_tile_shape(B, s);
+        double ms = 0.0;
+        if (!rc) rc = drp_batch_time_steps(B, 2, &ms);
+        if (!rc) rc = drp_batch_time_steps(B, nsteps, &ms);
+        if (rc) break;
+        ms /= nsteps;
+        if (ms_out) ms_out[s] = ms;
+        if (ms < bestms) {
+            bestms = ms;
+            *best = s;
+        }
+    }
+    drp_batch_destroy(B);
+    return rc;
 }
 
 extern "C" int drp_batch_sync(drp_batch* B) {

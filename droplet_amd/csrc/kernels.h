@@ -63,6 +63,8 @@ struct StepArgs {
     uint64_t* range;      // 3 slots x (max, min) as order-preserving bits
     int slot_read, slot_write, slot_reset;
     int wrap_mod;         // set by the launcher: a grid extent does not exceed the tile extent
+    int model;            // 0 free energy (phi populations), 1 Allen-Cahn (phi stencil update)
+    double ac_a, ac_m, ac_w;  // Allen-Cahn A, mobility M = (1 - A)/6, interface width W
 };
 
 hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st, const BatchLaunch* batch = nullptr);

@@ -108,6 +108,14 @@ static int validate(const drp_config* c) {
     }
     if (c->anu != 0.0)
         return drp_fail(DRP_ERR_UNSUPPORTED, "anu != 0 (velocity-gradient term of geq) is not implemented");
+    if (c->model != DRP_MODEL_FREE_ENERGY && c->model != DRP_MODEL_ALLEN_CAHN)
+        return drp_fail(DRP_ERR_ARG, "unknown model %d", c->model);
+    if (c->model == DRP_MODEL_ALLEN_CAHN) {
+        if (!(c->ac_width > 0.0)) return drp_fail(DRP_ERR_ARG, "Allen-Cahn interface width ac_width must be positive");
+        if (!(c->ac_a >= 0.0 && c->ac_a < 1.0)) return drp_fail(DRP_ERR_ARG, "Allen-Cahn ac_a must lie in [0, 1)");
+        if (c->precision != DRP_FP64 || c->force_kind != DRP_FORCE_NONE || (c->les_cs != 0.0 && c->les_delta != 0.0))
+            return drp_fail(DRP_ERR_UNSUPPORTED, "the Allen-Cahn model runs in FP64 without body force or LES");
+    }
     for (size_t k = 0; k < sizeof(c->reserved) / sizeof(c->reserved[0]); k++)
         if (c->reserved[k] != 0) return drp_fail(DRP_ERR_ARG, "reserved fields must be zero");
     {
@@ -329,6 +337,33 @@ extern "C" int drp_init_droplets(drp_lattice* L, const drp_drop* drops, int ndro
     return init_from_host_fields(L, f);
 }
 
+// Diffuse-interface droplets: phi = (phi1 + phi2)/2 + (phi2 - phi1)/2 tanh((R - r)/(sqrt2 W)),
+// r the distance to the centre, the largest value over the droplets at each node.
+extern "C" int drp_init_droplets_tanh(drp_lattice* L, const drp_drop* drops, int ndrops, double p_init,
+                                      double width) {
+    if (!L) return drp_fail(DRP_ERR_ARG, "null lattice");
+    if (ndrops < 1 || !drops) return drp_fail(DRP_ERR_ARG, "bad droplet list");
+    if (!(width > 0.0)) return drp_fail(DRP_ERR_ARG, "interface width must be positive");
+    const drp_config& c = L->cfg;
+    const int64_t n = L->n;
+    const double mid = 0.5 * (c.phi1 + c.phi2), half = 0.5 * (c.phi2 - c.phi1), s = std::sqrt(2.0) * width;
+    std::vector<double> phi(n), p(n, p_init), zero(n, 0.0);
+    for (int z = 0; z < c.nz; z++)
+        for (int y = 0; y < c.ny; y++)
+            for (int x = 0; x < c.nx; x++) {
+                double v = -1.0;
+                for (int k = 0; k < ndrops; k++) {
+                    const double dx = (double)x - drops[k].xc, dy = (double)y - drops[k].yc,
+                                 dz = (double)z - drops[k].zc;
+                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    v = std::fmax(v, std::tanh((0.5 * drops[k].diameter - r) / s));
+                }
+                phi[x + (int64_t)c.nx * (y + (int64_t)c.ny * z)] = mid + half * v;
+            }
+    const double* f[5] = {phi.data(), p.data(), zero.data(), zero.data(), zero.data()};
+    return init_from_host_fields(L, f);
+}
+
 extern "C" int drp_init_fields(drp_lattice* L, const double* phi, const double* p, const double* u1,
                                const double* u2, const double* u3) {
     if (!L || !phi || !p || !u1 || !u2 || !u3) return drp_fail(DRP_ERR_ARG, "null argument");
@@ -384,6 +419,10 @@ StepArgs make_step_args(drp_lattice* L, int64_t nstep) {
         a.fy = L->force + n;
         a.fz = L->force + 2 * n;
     }
+    a.model = c.model;
+    a.ac_a = c.ac_a;
+    a.ac_m = (1.0 - c.ac_a) * 1.0 / 6.0;  // M = (1 - A) ds / 6, dropac.f90:65
+    a.ac_w = c.ac_width;
     a.range = L->range;
     a.slot_read = (int)((nstep - 1) % 3);
     a.slot_write = (int)(nstep % 3);
@@ -401,7 +440,11 @@ extern "C" int drp_step(drp_lattice* L, int nsteps) {
         int rc = spectral_update(L, nstep);
         if (rc) return rc;
         StepArgs a = make_step_args(L, nstep);
-        HIP_TRY(launch_step_fp64(a, L->shape, L->stream));
+        const hipError_t e = launch_step_fp64(a, L->shape, L->stream);
+        if (e == hipErrorInvalidConfiguration)
+            return drp_fail(DRP_ERR_UNSUPPORTED, "tile shape %d: the staging of this step variant does not fit in LDS",
+                            L->shape);
+        HIP_TRY(e);
         L->cur = 1 - L->cur;
         L->step = nstep;
         rc = les_update_tau(L, false);
@@ -454,6 +497,8 @@ extern "C" int drp_get_macro(drp_lattice* L, int field, double* host, size_t n) 
 extern "C" int drp_get_populations(drp_lattice* L, int set, double* host, size_t n) {
     if (!L || !host) return drp_fail(DRP_ERR_ARG, "null argument");
     if (set != DRP_POP_PHI && set != DRP_POP_MOM) return drp_fail(DRP_ERR_ARG, "unknown population set %d", set);
+    if (set == DRP_POP_PHI && L->cfg.model == DRP_MODEL_ALLEN_CAHN)
+        return drp_fail(DRP_ERR_UNSUPPORTED, "the Allen-Cahn model has no phi populations");
     if (n != 15 * (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != 15 x %lld", n, (long long)L->n);
     if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
     if (L->cfg.precision != DRP_FP64) return fp32_get_populations(L, set, host);
@@ -469,6 +514,8 @@ extern "C" int drp_get_populations(drp_lattice* L, int set, double* host, size_t
 static int set_pops(drp_lattice* L, int set, const double* host, size_t n, int which, bool replace) {
     if (!L || !host) return drp_fail(DRP_ERR_ARG, "null argument");
     if (set != DRP_POP_PHI && set != DRP_POP_MOM) return drp_fail(DRP_ERR_ARG, "unknown population set %d", set);
+    if (set == DRP_POP_PHI && L->cfg.model == DRP_MODEL_ALLEN_CAHN)
+        return drp_fail(DRP_ERR_UNSUPPORTED, "the Allen-Cahn model has no phi populations");
     if (n != 15 * (size_t)L->n) return drp_fail(DRP_ERR_ARG, "size %zu != 15 x %lld", n, (long long)L->n);
     if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
     if (L->cfg.precision != DRP_FP64) {
@@ -550,6 +597,12 @@ extern "C" int drp_reduce(drp_lattice* L, drp_stats* st) {
             s[k] = t;
         }
     for (int k = 0; k < NPART; k++) s[k] += comp[k];
+    stats_from_sums(L, s, rg, st);
+    return DRP_OK;
+}
+
+// drp_stats from the compensated sums of the NPART diagnostics and the phi range slot
+void stats_from_sums(const drp_lattice* L, const double* s, const uint64_t* rg, drp_stats* st) {
     const double nn = (double)L->n;
     std::memset(st, 0, sizeof(*st));
     st->phi_mass = s[0];
@@ -564,7 +617,6 @@ extern "C" int drp_reduce(drp_lattice* L, drp_stats* st) {
     st->momentum[1] = s[7];
     st->momentum[2] = s[8];
     st->interaction = s[9];
-    return DRP_OK;
 }
 
 extern "C" int drp_deformation(drp_lattice* L, int mode, double* Lo, double* Bo, double* Do) {
@@ -616,6 +668,11 @@ extern "C" int drp_autotune(const drp_config* cfg, int nsteps, int* best, double
         L->shape = s;
         double ms = 0.0;
         rc = drp_time_steps(L, 2, &ms);
+        if (rc == DRP_ERR_UNSUPPORTED) {  // a shape this variant cannot use
+            if (ms_out) ms_out[s] = -1.0;
+            rc = DRP_OK;
+            continue;
+        }
         if (!rc) rc = drp_time_steps(L, nsteps, &ms);
         if (rc) break;
         ms /= nsteps;

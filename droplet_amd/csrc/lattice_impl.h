@@ -45,6 +45,8 @@ int default_shape(int precision);
 // SPECTRAL: recompute the force field from the current velocity (FP64 or FP32 fields)
 int spectral_update(drp_lattice* L, int64_t nstep);
 int reset_range(drp_lattice* L);
+// drp_stats from the compensated sums of the NPART diagnostics and the phi range slot
+void stats_from_sums(const drp_lattice* L, const double* sums, const uint64_t* range_slot, drp_stats* st);
 // LES tau for the next step: after a step (fresh = false, rank-block halos one
 // step old) or for a replaced state (fresh = true, all neighbours current)
 int les_update_tau(drp_lattice* L, bool fresh);

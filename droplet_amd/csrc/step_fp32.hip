@@ -398,17 +398,36 @@ static hipError_t launch32_variant(const StepArgs32& a, hipStream_t st, const Ba
     }
 }
 
+// Compiled once per tile shape (DRP_STEP_SHAPE = k) and once for the dispatcher,
+// as step_kernels.hip.
+template <int S> hipError_t launch_step_fp32_shape(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
+
+#if defined(DRP_STEP_SHAPE)
+template <>
+hipError_t launch_step_fp32_shape<DRP_STEP_SHAPE>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch) {
+    constexpr int S = DRP_STEP_SHAPE;
+    return launch32_variant<SHAPE_DIMS[S][0], SHAPE_DIMS[S][1], SHAPE_DIMS[S][2]>(a, st, batch);
+}
+#else
+template <> hipError_t launch_step_fp32_shape<0>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp32_shape<1>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp32_shape<2>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp32_shape<3>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp32_shape<4>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp32_shape<5>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp32_shape<6>(const StepArgs32& a, hipStream_t st, const BatchLaunch* batch);
 hipError_t launch_step_fp32(const StepArgs32& a, int shape, hipStream_t st, const BatchLaunch* batch) {
     switch (shape) {
-        case SHAPE_16x4x4: return launch32_variant<16, 4, 4>(a, st, batch);
-        case SHAPE_32x4x2: return launch32_variant<32, 4, 2>(a, st, batch);
-        case SHAPE_64x2x2: return launch32_variant<64, 2, 2>(a, st, batch);
-        case SHAPE_32x2x4: return launch32_variant<32, 2, 4>(a, st, batch);
-        case SHAPE_32x4x4: return launch32_variant<32, 4, 4>(a, st, batch);
-        case SHAPE_32x8x2: return launch32_variant<32, 8, 2>(a, st, batch);
-        case SHAPE_64x4x2: return launch32_variant<64, 4, 2>(a, st, batch);
+        case 0: return launch_step_fp32_shape<0>(a, st, batch);
+        case 1: return launch_step_fp32_shape<1>(a, st, batch);
+        case 2: return launch_step_fp32_shape<2>(a, st, batch);
+        case 3: return launch_step_fp32_shape<3>(a, st, batch);
+        case 4: return launch_step_fp32_shape<4>(a, st, batch);
+        case 5: return launch_step_fp32_shape<5>(a, st, batch);
+        case 6: return launch_step_fp32_shape<6>(a, st, batch);
         default: return hipErrorInvalidValue;
     }
 }
+#endif
 
 }  // namespace drp

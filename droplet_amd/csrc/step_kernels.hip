@@ -62,6 +62,10 @@ __device__ __forceinline__ double div_const(double x, double c, double r) {
 }
 
 enum { TAU_CONST = 0, TAU_PHI = 1, TAU_FIELD = 2 };
+// interface models: the free-energy model carries phi by populations; the
+// conservative Allen-Cahn model (prototype_program/two-phase/dropac.f90) has
+// none and advances phi by a stencil of local equilibria
+enum { MODEL_FE = 0, MODEL_AC = 1 };
 
 // Resident wavefronts per SIMD the register allocation targets: two 8-wavefront
 // workgroups per CU fit the LDS budget, i.e. 4 per SIMD (<= 128 VGPRs).  The
@@ -76,7 +80,7 @@ template <bool WALL, int FORCE> constexpr int waves_per_eu() { return (WALL && F
 // workgroups of `bytes` of LDS that fit in one CU (160 KiB, 1280-byte granules)
 constexpr int wg_per_cu_lds(int bytes) { return (160 * 1024) / (((bytes + 1279) / 1280) * 1280); }
 
-template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE, int MODEL = MODEL_FE>
 __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     constexpr int NN = TX * TY * TZ, NT = 2 * NN;
     constexpr int HX2 = TX + 4, HY2 = TY + 4, HZ2 = TZ + 4, H2 = HX2 * HY2 * HZ2;
@@ -90,9 +94,14 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     // same operations the per-direction equilibria would repeat (bit-identical),
     // each only where it does not lower the number of workgroups that fit in a
     // CU's 160 KiB of LDS (sizes rounded up to a 1280-byte allocation granule)
-    constexpr int LDS_BASE = 8 * (H2 + 8 * H1 + (TAUMODE == TAU_CONST ? 1 : H1) + 2 * (NT / 64));
-    constexpr bool PRE_S = wg_per_cu_lds(LDS_BASE + 8 * H1) >= wg_per_cu_lds(LDS_BASE);
-    constexpr bool PRE_U = PRE_S && wg_per_cu_lds(LDS_BASE + 16 * H1) >= wg_per_cu_lds(LDS_BASE);
+    // Allen-Cahn: the unit normal n = grad phi / (|grad phi| + eps) per halo node
+    // in s_A (n0), s_n1, s_n2 (the F-bracket has no role in that model)
+    __shared__ double s_n1[MODEL == MODEL_AC ? H1 : 1], s_n2[MODEL == MODEL_AC ? H1 : 1];
+    constexpr int LDS_BASE = 8 * (H2 + 8 * H1 + (TAUMODE == TAU_CONST ? 1 : H1) + 2 * (NT / 64) +
+                                  (MODEL == MODEL_AC ? 2 * H1 : 0));
+    constexpr bool PRE_S = LDS_BASE + 8 * H1 <= 160 * 1024 && wg_per_cu_lds(LDS_BASE + 8 * H1) >= wg_per_cu_lds(LDS_BASE);
+    constexpr bool PRE_U =
+        PRE_S && LDS_BASE + 16 * H1 <= 160 * 1024 && wg_per_cu_lds(LDS_BASE + 16 * H1) >= wg_per_cu_lds(LDS_BASE);
     __shared__ double s_s2h[PRE_S ? H1 : 1], s_usq[PRE_U ? H1 : 1];
     __shared__ double s_tau[TAUMODE == TAU_CONST ? 1 : H1];
     __shared__ double s_red[2][NT / 64];
@@ -159,12 +168,15 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const int64_t zo[3] = {nxy * zs[0], nxy * zs[1], nxy * zs[2]};
     const double* __restrict__ pin = mom ? a.g_in : a.f_in;
     // unconditional: threads outside the grid read node (0, 0, 0)'s neighbours
+    // (the Allen-Cahn model has no phi populations to load)
     double ps[15];
-    static_for<0, 15>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        using D = Dir<q>;
-        ps[q] = __builtin_nontemporal_load(&pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]]);
-    });
+    if (MODEL == MODEL_FE || mom) {
+        static_for<0, 15>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            using D = Dir<q>;
+            ps[q] = __builtin_nontemporal_load(&pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]]);
+        });
+    }
 
     // ---------------- phase 1c: stage the fields in LDS
     static_for<0, IT2>([&](auto kc) {
@@ -207,12 +219,20 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             g1 = acc<D::cy>(g1, v);
             g2 = acc<D::cz>(g2, v);
         });
-        lap = div_const(lap, 5.0, 0.2);
         g0 = div_const(g0, 10.0, 0.1);
         g1 = div_const(g1, 10.0, 0.1);
         g2 = div_const(g2, 10.0, 0.1);
-        const double p0 = ph * a.T / (1.0 - a.b * ph) - a.a * (ph * ph);
-        s_A[h] = p0 - a.kf * ph * lap - a.kf6 * (g0 * g0 + g1 * g1 + g2 * g2);
+        if constexpr (MODEL == MODEL_AC) {
+            // n_vector_cal, dropac.f90:715-728 (eps = 1e-12, :67)
+            const double nrm = sqrt(g0 * g0 + g1 * g1 + g2 * g2) + 1.0e-12;
+            s_A[h] = g0 / nrm;
+            s_n1[h] = g1 / nrm;
+            s_n2[h] = g2 / nrm;
+        } else {
+            lap = div_const(lap, 5.0, 0.2);
+            const double p0 = ph * a.T / (1.0 - a.b * ph) - a.a * (ph * ph);
+            s_A[h] = p0 - a.kf * ph * lap - a.kf6 * (g0 * g0 + g1 * g1 + g2 * g2);
+        }
         s_g0[h] = g0;
         if constexpr (PRE_S) s_s2h[h] = 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
         if constexpr (PRE_U) s_usq[h] = s_u1[h] * s_u1[h] + s_u2[h] * s_u2[h] + s_u3[h] * s_u3[h];
@@ -231,7 +251,32 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     double vmax = -__builtin_huge_val(), vmin = __builtin_huge_val();
     if (inside) {
         const int64_t s = x + (int64_t)nx * (y + (int64_t)ny * z);
-        if (!mom) {
+        if (!mom && MODEL == MODEL_AC) {
+            // physics, dropac.f90:640-667: phi(x) <- sum_i [feq_i(x - c_i) + A E_i (phi(x) - phi(x - c_i))]
+            // with feq_i = E_i phi (1 + 3 c.u) + 3 E_i M (1 - 4 phi^2) (c.n) / W (feq_cal :517-535),
+            // all at the step's start, in the reference's order of operations
+            const double phx = s_phi[(tx + 2) + HX2 * ((ty + 2) + HY2 * (tz + 2))];
+            double ph = 0.0;
+            static_for<0, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                constexpr double E = weightE<q>();
+                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                const int h2 = (tx + 2 - D::cx) + HX2 * ((ty + 2 - D::cy) + HY2 * (tz + 2 - D::cz));
+                const double phu = s_phi[h2];
+                double fe = E * phu;
+                if constexpr (q != 0) {
+                    const double cu = cdot<q>(s_u1[hu], s_u2[hu], s_u3[hu]);
+                    const double cn = cdot<q>(s_A[hu], s_n1[hu], s_n2[hu]);
+                    fe = fe * (1.0 + 3.0 * cu) + (3.0 * E) * a.ac_m * (1.0 - 4.0 * (phu * phu)) * cn / a.ac_w;
+                }
+                ph = ph + fe + (a.ac_a * E) * (phx - phu);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            __builtin_nontemporal_store(ph, &a.mac_out[s]);
+            vmax = ph;
+            vmin = ph;
+        } else if (!mom) {
             double ph = 0.0;
             static_for<0, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
@@ -396,9 +441,9 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
 }
 
 // one lattice: workgroup id -> tile (XCD-contiguous)
-template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE, int MODEL>
 __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu<WALL, FORCE>()))) void step_fp64(StepArgs a) {
-    step_tile_fp64<TX, TY, TZ, WALL, FORCE, TAUMODE>(a, xcd_tile(blockIdx.x, gridDim.x));
+    step_tile_fp64<TX, TY, TZ, WALL, FORCE, TAUMODE, MODEL>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // many independent lattices in one launch: tile_start[i] is the first global
@@ -417,8 +462,11 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(
 }
 
 // ------------------------------------------------------------------ launcher
-template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUM>
+template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUM, int MODEL = MODEL_FE>
 static hipError_t launch_shape(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
+    if constexpr (MODEL != MODEL_FE) {
+        if (batch) return hipErrorInvalidValue;  // ensembles run the free-energy model only
+    }
     if (batch) {
         hipLaunchKernelGGL((step_batch_fp64<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(batch->total_tiles),
                            dim3(2 * TX * TY * TZ), 0, st, (const StepArgs*)batch->args, batch->tile_start,
@@ -428,7 +476,7 @@ static hipError_t launch_shape(const StepArgs& a, hipStream_t st, const BatchLau
     StepArgs b = a;
     set_tiles(b, TX, TY, TZ);
     const int nb = b.tiles_x * b.tiles_y * b.tiles_z;
-    hipLaunchKernelGGL((step_fp64<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(nb), dim3(2 * TX * TY * TZ), 0, st, b);
+    hipLaunchKernelGGL((step_fp64<TX, TY, TZ, WALL, FORCE, TAUM, MODEL>), dim3(nb), dim3(2 * TX * TY * TZ), 0, st, b);
     return hipGetLastError();
 }
 
@@ -439,9 +487,29 @@ static hipError_t launch_tau(const StepArgs& a, hipStream_t st, const BatchLaunc
     return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_PHI>(a, st, batch);
 }
 
+// LDS bytes of the Allen-Cahn variant without the optional per-node tables:
+// phi over the 2-halo box, p, u, n (3), grad phi (3) over the 1-halo box, tau if variable
+constexpr int ac_lds_bytes(int TX, int TY, int TZ, bool var_tau) {
+    return 8 * ((TX + 4) * (TY + 4) * (TZ + 4) + (10 + (var_tau ? 1 : 0)) * (TX + 2) * (TY + 2) * (TZ + 2) +
+                (var_tau ? 0 : 1) + 2 * (2 * TX * TY * TZ / 64));
+}
+
 template <int TX, int TY, int TZ>
 static hipError_t launch_variant(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
     const bool wall = a.wall_axis != 0;
+    if (a.model == MODEL_AC) {  // no body force, no LES; tile shapes whose staging fits in LDS
+        if (a.force_kind != FORCE_NONE || a.tau_field) return hipErrorInvalidValue;
+        if (a.dnu == 0.0) {
+            if constexpr (ac_lds_bytes(TX, TY, TZ, false) <= 160 * 1024)
+                return wall ? launch_shape<TX, TY, TZ, true, FORCE_NONE, TAU_CONST, MODEL_AC>(a, st, batch)
+                            : launch_shape<TX, TY, TZ, false, FORCE_NONE, TAU_CONST, MODEL_AC>(a, st, batch);
+            return hipErrorInvalidConfiguration;
+        }
+        if constexpr (ac_lds_bytes(TX, TY, TZ, true) <= 160 * 1024)
+            return wall ? launch_shape<TX, TY, TZ, true, FORCE_NONE, TAU_PHI, MODEL_AC>(a, st, batch)
+                        : launch_shape<TX, TY, TZ, false, FORCE_NONE, TAU_PHI, MODEL_AC>(a, st, batch);
+        return hipErrorInvalidConfiguration;
+    }
     switch (a.force_kind) {
         case FORCE_NONE:
             return wall ? launch_tau<TX, TY, TZ, true, FORCE_NONE>(a, st, batch) : launch_tau<TX, TY, TZ, false, FORCE_NONE>(a, st, batch);
@@ -455,17 +523,38 @@ static hipError_t launch_variant(const StepArgs& a, hipStream_t st, const BatchL
     }
 }
 
+// The build compiles this file once per tile shape (DRP_STEP_SHAPE = k, which
+// instantiates that shape's variants as launch_step_fp64_shape<k>) and once
+// without it for the dispatcher, so the shapes compile in parallel.
+template <int S> hipError_t launch_step_fp64_shape(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
+
+#if defined(DRP_STEP_SHAPE)
+template <>
+hipError_t launch_step_fp64_shape<DRP_STEP_SHAPE>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
+    constexpr int S = DRP_STEP_SHAPE;
+    return launch_variant<SHAPE_DIMS[S][0], SHAPE_DIMS[S][1], SHAPE_DIMS[S][2]>(a, st, batch);
+}
+#else
+template <> hipError_t launch_step_fp64_shape<0>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp64_shape<1>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp64_shape<2>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp64_shape<3>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp64_shape<4>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp64_shape<5>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
+template <> hipError_t launch_step_fp64_shape<6>(const StepArgs& a, hipStream_t st, const BatchLaunch* batch);
 hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st, const BatchLaunch* batch) {
     switch (shape) {
-        case SHAPE_16x4x4: return launch_variant<16, 4, 4>(a, st, batch);
-        case SHAPE_32x4x2: return launch_variant<32, 4, 2>(a, st, batch);
-        case SHAPE_64x2x2: return launch_variant<64, 2, 2>(a, st, batch);
-        case SHAPE_32x2x4: return launch_variant<32, 2, 4>(a, st, batch);
-        case SHAPE_32x4x4: return launch_variant<32, 4, 4>(a, st, batch);
-        case SHAPE_32x8x2: return launch_variant<32, 8, 2>(a, st, batch);
-        case SHAPE_64x4x2: return launch_variant<64, 4, 2>(a, st, batch);
+        case 0: return launch_step_fp64_shape<0>(a, st, batch);
+        case 1: return launch_step_fp64_shape<1>(a, st, batch);
+        case 2: return launch_step_fp64_shape<2>(a, st, batch);
+        case 3: return launch_step_fp64_shape<3>(a, st, batch);
+        case 4: return launch_step_fp64_shape<4>(a, st, batch);
+        case 5: return launch_step_fp64_shape<5>(a, st, batch);
+        case 6: return launch_step_fp64_shape<6>(a, st, batch);
         default: return hipErrorInvalidValue;
     }
 }
+#endif
+static_assert(SHAPE_COUNT == 7, "one dispatcher case per tile shape");
 
 }  // namespace drp

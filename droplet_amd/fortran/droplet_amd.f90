@@ -34,7 +34,9 @@ module droplet_amd
         integer(c_int32_t) :: force_kind, force_start
         real(c_double) :: force_amp, force_k, les_cs, les_delta
         integer(c_int32_t) :: les_ranks_x, les_ranks_y, les_ranks_z
-        integer(c_int32_t) :: reserved(5)
+        integer(c_int32_t) :: model
+        integer(c_int32_t) :: reserved(4)
+        real(c_double) :: ac_a, ac_width
     end type drp_config
 
     type, bind(C) :: drp_drop
@@ -287,7 +289,7 @@ contains
         type(drp_stats) :: s
         integer(c_int) :: v
         v = drp_abi_sizes(sc, sd, ss)
-        drp_layout_ok = (v == 1) .and. (sc == c_sizeof(c)) .and. (sd == c_sizeof(d)) .and. (ss == c_sizeof(s))
+        drp_layout_ok = (v == 2) .and. (sc == c_sizeof(c)) .and. (sd == c_sizeof(d)) .and. (ss == c_sizeof(s))
     end function
 
 end module droplet_amd

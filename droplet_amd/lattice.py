@@ -45,6 +45,7 @@ def lib():
         L.drp_destroy.restype = None
         L.drp_init_droplets.argtypes = [vp, C.POINTER(Drop), C.c_int, C.c_double]
         L.drp_init_fields.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.drp_init_droplets_tanh.argtypes = [vp, C.POINTER(Drop), C.c_int, C.c_double, C.c_double]
         L.drp_step.argtypes = [vp, C.c_int]
         L.drp_sync.argtypes = [vp]
         L.drp_step_count.argtypes = [vp]
@@ -78,6 +79,12 @@ def lib():
         L.drp_batch_set_tile_shape.argtypes = [vp, C.c_int]
         L.drp_batch_sync.argtypes = [vp]
         L.drp_batch_time_steps.argtypes = [vp, C.c_int, dp]
+        L.drp_batch_autotune.argtypes = [C.POINTER(Config), C.c_int, C.c_int, C.POINTER(C.c_int), dp]
+        L.drp_copy_bandwidth_kind.argtypes = [C.c_size_t, C.c_int, C.c_int, dp]
+        L.drp_energy_spectrum.argtypes = [vp, dp, C.c_int]
+        L.drp_scale_interaction.argtypes = [vp, dp, C.c_int]
+        L.drp_filter_velocity.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, sz]
+        L.drp_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int]
         L.drp_batch_destroy.argtypes = [vp]
         L.drp_batch_destroy.restype = None
         L.drp_tape_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
@@ -135,6 +142,11 @@ class Lattice:
     def init_droplets(self, drops, p_init: float = 1.0 / 3.0):
         arr = (Drop * max(1, len(drops)))(*[Drop(*d) for d in drops])
         _check(lib().drp_init_droplets(self._h, arr, len(drops), p_init), "drp_init_droplets")
+
+    def init_droplets_tanh(self, drops, width: float, p_init: float = 1.0 / 3.0):
+        """Diffuse-interface droplets, phi = mid + half tanh((D/2 - r)/(sqrt2 width))."""
+        arr = (Drop * len(drops))(*[Drop(*d) for d in drops])
+        _check(lib().drp_init_droplets_tanh(self._h, arr, len(drops), p_init, width), "drp_init_droplets_tanh")
 
     def init_fields(self, phi, p, u1, u2, u3):
         fs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (phi, p, u1, u2, u3)]
@@ -221,6 +233,27 @@ class Lattice:
         _check(lib().drp_deformation(self._h, int(mode), C.byref(L), C.byref(B), C.byref(D)), "drp_deformation")
         return L.value, B.value, D.value
 
+    def energy_spectrum(self, nshells: int | None = None) -> np.ndarray:
+        """E[s] of the shells int|k| = s (drp_energy_spectrum; default nx/2 + 1 shells)."""
+        ns = self.cfg.nx // 2 + 1 if nshells is None else int(nshells)
+        out = np.zeros(ns)
+        _check(lib().drp_energy_spectrum(self._h, _dp(out), ns), "drp_energy_spectrum")
+        return out
+
+    def scale_interaction(self, nshells: int | None = None) -> np.ndarray:
+        """kappa_g <u^(s) . div(grad phi grad phi)> per shell s (drp_scale_interaction)."""
+        ns = self.cfg.nx // 2 + 2 if nshells is None else int(nshells)
+        out = np.zeros(ns)
+        _check(lib().drp_scale_interaction(self._h, _dp(out), ns), "drp_scale_interaction")
+        return out
+
+    def filter_velocity(self, kind: int, k: int):
+        """(u1, u2, u3) low-passed (kind 0: shells < k) or band-passed (kind 1: shell k)."""
+        out = [np.empty(self.shape, np.float64) for _ in range(3)]
+        _check(lib().drp_filter_velocity(self._h, int(kind), int(k), *[_dp(a) for a in out], self.n),
+               "drp_filter_velocity")
+        return tuple(out)
+
     def droplet_census(self, threshold: float | None = None, cap: int = 1 << 16, labels: bool = False):
         """Droplets of the current state (drp_droplet_census): (first, volumes[, labels]),
         ascending in first; threshold defaults to (phi1 + phi2) / 2 as in
@@ -254,6 +287,32 @@ def autotune(cfg: Config, nsteps: int = 10):
     return best.value, list(ms)
 
 
+def batch_autotune(cfgs, nsteps: int = 10):
+    """Fastest tile shape of the batched step for these members; returns (best, [ms per step per shape])."""
+    n = tile_shape_count()
+    best = C.c_int(-1)
+    ms = (C.c_double * n)()
+    arr = (Config * len(cfgs))(*cfgs)
+    _check(lib().drp_batch_autotune(arr, len(cfgs), int(nsteps), C.byref(best), ms), "drp_batch_autotune")
+    return best.value, list(ms)
+
+
+def copy_bandwidth_kind(nbytes: int, kind: int, reps: int = 20) -> float:
+    """GB/s of yardstick `kind` (0 grid-stride copy, 1 hipMemcpyDtoD, 2 unrolled non-temporal
+    copy, 3 unrolled plain copy, 4 read only, 5 write only) over nbytes."""
+    gbs = C.c_double()
+    _check(lib().drp_copy_bandwidth_kind(int(nbytes), int(kind), int(reps), C.byref(gbs)), "drp_copy_bandwidth_kind")
+    return gbs.value
+
+
+def fft3d(a: np.ndarray, sign: int = -1) -> np.ndarray:
+    """The library's 3-D FFT of a complex [z][y][x] array (unnormalised)."""
+    buf = np.ascontiguousarray(a, np.complex128).copy()
+    nz, ny, nx = buf.shape
+    _check(lib().drp_fft3d(nx, ny, nz, buf.ctypes.data_as(C.POINTER(C.c_double)), int(sign)), "drp_fft3d")
+    return buf
+
+
 def copy_bandwidth_soa(bytes_per_stream: int, streams: int = 40, reps: int = 10) -> float:
     """GB/s of `streams` SoA arrays copied at once (read + write counted)."""
     gbs = C.c_double()
@@ -272,7 +331,13 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
 class Batch:
     """Independent lattices (any sizes and parameters) advanced by one launch per step."""
 
-    def __init__(self, cfgs):
+    def __init__(self, cfgs, tile_shape=None, tune_steps: int = 10):
+        """tile_shape: None keeps the library default, an int selects a shape, "auto"
+        runs drp_batch_autotune on these members first (self.tuned holds its timings)."""
+        self.tuned = None
+        if tile_shape == "auto":
+            tile_shape, ms = batch_autotune(cfgs, tune_steps)
+            self.tuned = ms
         arr = (Config * len(cfgs))(*cfgs)
         h = C.c_void_p()
         _check(lib().drp_batch_create(arr, len(cfgs), C.byref(h)), "drp_batch_create")
@@ -286,6 +351,9 @@ class Batch:
             m._h = C.c_void_p(lib().drp_batch_member(h, i))
             m._borrowed = True
             self.members.append(m)
+        self.tile_shape_used = tile_shape
+        if tile_shape is not None:
+            self.set_tile_shape(int(tile_shape))
 
     def __len__(self):
         return len(self.members)

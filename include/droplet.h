@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define DROPLET_ABI_VERSION 1
+#define DROPLET_ABI_VERSION 2
 
 #define DRP_OK 0
 #define DRP_ERR_ARG (-1)         /* invalid argument / size mismatch */
@@ -59,6 +59,15 @@ extern "C" {
 #define DRP_FORCE_SPECTRAL 3     /* linear forcing of the shells 1 <= |k| < force_k with
                                     injection rate force_amp, recomputed from u every step
                                     (externalforce, program/lbm_drop_iht_xy.f90:722-841) */
+
+/* interface models */
+#define DRP_MODEL_FREE_ENERGY 0 /* Inamuro free-energy model: phi carried by populations
+                                   (program/lbm_drop_xy.f90)                          */
+#define DRP_MODEL_ALLEN_CAHN 1  /* conservative Allen-Cahn phase field: phi advanced by a
+                                   stencil of local equilibria with mobility M = (1-A)/6,
+                                   interface width W, no phi populations
+                                   (prototype_program/two-phase/dropac.f90:517-535,640-667,
+                                   715-728); FP64, no LES, no body force                */
 
 /* macroscopic fields for drp_get_macro / drp_set_macro */
 #define DRP_FIELD_PHI 0
@@ -99,7 +108,11 @@ typedef struct drp_config {
     int32_t les_ranks_x;
     int32_t les_ranks_y;
     int32_t les_ranks_z;
-    int32_t reserved[5];  /* must be zero                                              */
+    int32_t model;        /* DRP_MODEL_*                                               */
+    int32_t reserved[4];  /* must be zero                                              */
+    double ac_a;          /* Allen-Cahn: relaxation weight A of the phi update; mobility
+                             M = (1 - A)/6 (ref dropac.f90 A, M)                       */
+    double ac_width;      /* Allen-Cahn: interface width W (ref dropac.f90 W)          */
 } drp_config;
 
 typedef struct drp_drop {
@@ -117,7 +130,11 @@ typedef struct drp_stats {
     double p_mean;        /* <p>                                                       */
     double momentum[3];   /* sum u                                                     */
     double interaction;   /* sum over nodes of sum_ab d_a phi d_b phi S_ab, S = (grad u + grad u^T)/2:
-                             the interface-strain interaction of ref drop_analysis.f90:797-882 */
+                             a restatement of grad_u_cal / strain_cal / interaction_cal of ref
+                             drop_analysis.f90:797-882, routines whose calls the reference has
+                             commented out (:1271-1274); its parity is unpinned (closed forms
+                             only).  The form the reference runs is the per-scale
+                             kappa_g <u_k . div(grad phi grad phi)> of drp_scale_interaction. */
 } drp_stats;
 
 typedef struct drp_lattice drp_lattice; /* opaque; owns all device memory */
@@ -137,6 +154,12 @@ int drp_create(const drp_config* cfg, drp_lattice** out);
  * then f := feq, g := geq and the nu(phi) range (phi1, phi2)
  * (program/lbm_drop_xy.f90:224-284; prototype_program/two-phase/dropch.f90:186-473). */
 int drp_init_droplets(drp_lattice* lat, const drp_drop* drops, int ndrops, double p_init);
+
+/* Diffuse-interface variant of drp_init_droplets (no reference counterpart: the
+ * reference starts from the sharp step): phi = (phi1+phi2)/2 + (phi2-phi1)/2 *
+ * tanh((D/2 - r)/(sqrt2 * width)), r the distance to a droplet centre, the largest
+ * value over the droplets; u = 0, p = p_init, populations := equilibria. */
+int drp_init_droplets_tanh(drp_lattice* lat, const drp_drop* drops, int ndrops, double p_init, double width);
 
 /* = ini from arbitrary host fields (each nx*ny*nz doubles): populations := equilibria. */
 int drp_init_fields(drp_lattice* lat, const double* phi, const double* p,
@@ -216,6 +239,12 @@ drp_lattice* drp_batch_member(drp_batch* b, int i);
 int drp_batch_step(drp_batch* b, int nsteps);
 int drp_batch_use_graph(drp_batch* b, int enable);
 int drp_batch_set_tile_shape(drp_batch* b, int shape);
+/* Autotuner for ensembles: times every tile shape of the batched step on a
+ * temporary batch of these members (each with a centred droplet), nsteps steps
+ * per shape, and returns the fastest; ms_per_step (may be NULL) receives
+ * drp_tile_shape_count() timings.  The caller applies it with
+ * drp_batch_set_tile_shape. */
+int drp_batch_autotune(const drp_config* cfgs, int count, int nsteps, int* best_shape, double* ms_per_step);
 int drp_batch_sync(drp_batch* b);
 int drp_batch_time_steps(drp_batch* b, int nsteps, double* ms);
 void drp_batch_destroy(drp_batch* b);
@@ -280,6 +309,12 @@ int drp_get_force(drp_lattice* lat, double* fx, double* fy, double* fz, size_t n
  * averaged over reps launches on a private stream. */
 int drp_copy_bandwidth(size_t bytes, int reps, double* gbs);
 
+/* Independent yardsticks for the same `bytes`: kind 0 the grid-stride copy above,
+ * 1 hipMemcpyDtoDAsync, 2 a one-pass copy with four 16-byte non-temporal loads
+ * in flight per lane before its stores, 3 the same with ordinary loads and
+ * stores, 4 read only (bytes read), 5 write only (bytes written). */
+int drp_copy_bandwidth_kind(size_t bytes, int kind, int reps, double* gbs);
+
 /* The same for `streams` (10, 20 or 40) structure-of-arrays streams of
  * bytes_per_stream each, copied at once with non-temporal stores: the step
  * kernel's access pattern without its arithmetic. */
@@ -298,6 +333,38 @@ int drp_copy_bandwidth_soa(size_t bytes_per_stream, int streams, int reps, doubl
    cross a wrap the counts agree.  Needs fewer than 2^30 nodes. */
 int drp_droplet_census(drp_lattice* lat, double threshold, int64_t* count, int64_t* volumes, int64_t* first,
                        int64_t cap, int32_t* labels, size_t n);
+
+/* ---- FFT diagnostics of the velocity field (extents products of 2, 3 and 5, at
+ * most 2048 per axis).  u_hat = FFT(u) / (nx ny nz); signed wavenumbers
+ * k = i - n for i >= n/2 + 1; shell s holds the modes with int(|k|) = s. ---- */
+
+/* Energy spectrum = enesupe_cal (program/turbulence_hit_xy.f90:588-644):
+ * E[s] = sum over the half space k3 >= 0 (index 0..nz/2) of
+ * 0.5 (|u1_hat|^2 + |u2_hat|^2 + |u3_hat|^2), weighted 2 off the k3 = 0 plane
+ * (the Nyquist plane k3 = nz/2 included, as the reference counts it), for
+ * s = 0 .. nshells-1 (the reference's enesupe(s+1), an array of nx/2 + 1;
+ * modes of higher shells, which it writes past that array, are dropped here). */
+int drp_energy_spectrum(drp_lattice* lat, double* E, int nshells);
+
+/* Spectral filters of the velocity (results to host, x fastest):
+ * DRP_FILTER_LOW_PASS keeps the shells int|k| < k (low_pass with k_low = k,
+ * program/LES_drop_xy.f90:851-971); DRP_FILTER_SHELL keeps int|k| == k (the
+ * band-pass of scale_cal2, program/drop_analysis.f90:722-797). */
+#define DRP_FILTER_LOW_PASS 0
+#define DRP_FILTER_SHELL 1
+int drp_filter_velocity(drp_lattice* lat, int kind, int k, double* u1, double* u2, double* u3, size_t n);
+
+/* Per-scale interface-velocity interaction (program/drop_analysis.f90:1121-1265,
+ * contribution_each_scale): C[s] = kappa_g < u^(s) . w >, the mean over the nodes
+ * of the shell-s band-pass of u dotted with w_b = sum_a d_a (d_b phi d_a phi)
+ * (D3Q15 gradients), for s = 0 .. nshells-1 (the reference evaluates s = 1 ..
+ * nx/2 + 1).  Evaluated by Parseval from one transform of u and of w. */
+int drp_scale_interaction(drp_lattice* lat, double* C, int nshells);
+
+/* The 3-D FFT these use, on a host array of nx*ny*nz complex numbers stored as
+ * interleaved (re, im), x fastest; sign -1 forward, +1 inverse, unnormalised.
+ * No reference counterpart (FFTW's role there); for checking the transform. */
+int drp_fft3d(int nx, int ny, int nz, double* data, int sign);
 
 /* Message for the calling thread's last failure ("" if none). */
 const char* drp_last_error(void);

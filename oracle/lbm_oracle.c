@@ -9,6 +9,7 @@
  *   program/lbm_drop_xy.f90            (periodic step, body force)
  *   prototype_program/two-phase/dropch.f90 (moving-wall bounce-back, break_up3)
  *   program/LES_drop_xy.f90            (Smagorinsky eddy viscosity)
+ *   prototype_program/two-phase/dropac.f90 (conservative Allen-Cahn model)
  * Every arithmetic expression keeps the reference's evaluation order (Fortran
  * evaluates a chain of + or * left to right), so that on the same inputs the
  * restatement and the compiled reference round identically.
@@ -87,6 +88,9 @@ ora_lattice* ora_create(const drp_config* c) {
         if (c->les_ranks_x < 1 || c->les_ranks_y < 1 || c->les_ranks_z < 1) return NULL;
         if (c->nx % c->les_ranks_x || c->ny % c->les_ranks_y || c->nz % c->les_ranks_z) return NULL;
     }
+    if (c->model == DRP_MODEL_ALLEN_CAHN &&
+        (c->force_kind != DRP_FORCE_NONE || (c->les_cs != 0.0 && c->les_delta != 0.0) || !(c->ac_width > 0.0)))
+        return NULL;
     ora_lattice* L = (ora_lattice*)calloc(1, sizeof(ora_lattice));
     if (!L) return NULL;
     L->c = *c;
@@ -234,11 +238,41 @@ static double velocity_gradient_contraction(const ora_lattice* L, int64_t s, int
     return acc;
 }
 
+static int allen_cahn(const ora_lattice* L) { return L->c.model == DRP_MODEL_ALLEN_CAHN; }
+
+/* Allen-Cahn local equilibrium of the phi update at one node and direction:
+   feq_cal, prototype_program/two-phase/dropac.f90:517-535, with the unit normal
+   n = grad phi / (|grad phi| + eps) of n_vector_cal (:715-728), eps = 1e-12 (:67),
+   M = (1 - A) ds / 6 (:65) */
+static double allen_cahn_feq(const ora_lattice* L, int64_t s, int i) {
+    const drp_config* c = &L->c;
+    const double phi = L->phi[s];
+    const double g0 = L->gp[s], g1 = L->gp[L->n + s], g2 = L->gp[2 * L->n + s];
+    const double nrm = sqrt(g0 * g0 + g1 * g1 + g2 * g2) + 1.0e-12;
+    const double n0 = g0 / nrm, n1 = g1 / nrm, n2 = g2 / nrm;
+    const double M = (1.0 - c->ac_a) * 1.0 / 6.0;
+    const double cu = (double)CX[i] * L->u1[s] + (double)CY[i] * L->u2[s] + (double)CZ[i] * L->u3[s];
+    const double cn = (double)CX[i] * n0 + (double)CY[i] * n1 + (double)CZ[i] * n2;
+    return EW[i] * phi * (1.0 + 3.0 * cu) + 3.0 * EW[i] * M * (1.0 - 4.0 * (phi * phi)) * cn / c->ac_width;
+}
+
 /* feq_i, geq_i at one node: equilibrium_cal, program/lbm_drop_xy.f90:421-476 */
 static void equilibria_at(ora_lattice* L, int64_t s) {
     const drp_config* c = &L->c;
     double phi = L->phi[s], p = L->p[s], u1 = L->u1[s], u2 = L->u2[s], u3 = L->u3[s];
     double g[3] = {L->gp[s], L->gp[L->n + s], L->gp[2 * L->n + s]};
+    if (allen_cahn(L)) {
+        /* feq_cal + geq_cal, dropac.f90:517-565: the same momentum equilibria */
+        double usq = u1 * u1 + u2 * u2 + u3 * u3, G[3][3];
+        korteweg_tensor(g, G);
+        for (int i = 0; i < 15; i++) {
+            L->feq[i * L->n + s] = allen_cahn_feq(L, s, i);
+            double gt = korteweg_contraction(G, i);
+            double cu = u1 * (double)CX[i] + u2 * (double)CY[i] + u3 * (double)CZ[i];
+            L->geq[i * L->n + s] = EW[i] * (3.0 * p + 3.0 * cu + 4.5 * (cu * cu) - 1.5 * usq) + EW[i] * c->kappa_g * gt;
+        }
+        return;
+    }
     double p0 = phi * c->T / (1.0 - c->b * phi) - c->a * (phi * phi);         /* :606 */
     double bracket = p0 - c->kappa_f * phi * L->lap[s]
                      - c->kappa_f / 6.0 * (g[0] * g[0] + g[1] * g[1] + g[2] * g[2]); /* :453-455 */
@@ -457,8 +491,9 @@ static void stream_collide(ora_lattice* L, int force_on) {
                 for (int i = 0; i < 15; i++) {
                     int64_t u = upstream(L, x, y, z, i);
                     double fs = L->f[i * n + u], gs = L->g[i * n + u];
-                    /* fg_cal, program/lbm_drop_xy.f90:497-508 */
-                    L->fn[i * n + s] = fs - (fs - L->feq[i * n + u]) / c->tau_f;
+                    /* fg_cal, program/lbm_drop_xy.f90:497-508 (Allen-Cahn: g_cal only,
+                       dropac.f90:588-603) */
+                    if (!allen_cahn(L)) L->fn[i * n + s] = fs - (fs - L->feq[i * n + u]) / c->tau_f;
                     double gv = gs - (gs - L->geq[i * n + u]) / L->tau[u];
                     if (force_on)
                         gv = gv + (ORACLE_PLANT == 2 ? -3.0 : 3.0) * EW[i]
@@ -493,6 +528,28 @@ static void bounce_back(ora_lattice* L) {
     }
 }
 
+/* Allen-Cahn phi update: physics, dropac.f90:640-667, phi(x) <- sum_i [feq_i(x - c_i)
+   + A E_i (phi(x) - phi(x - c_i))] from the equilibria and phi of the step's start */
+static void allen_cahn_phi(ora_lattice* L) {
+    const drp_config* c = &L->c;
+    const int nx = c->nx, ny = c->ny, nz = c->nz;
+    const int64_t n = L->n;
+    double* next = L->fn;  /* the phi populations' buffers are unused by this model */
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                int64_t s = idx(L, x, y, z);
+                double ph = 0.0;
+                for (int i = 0; i < 15; i++) {
+                    int64_t u = upstream(L, x, y, z, i);
+                    ph = ph + L->feq[i * n + u] + c->ac_a * EW[i] * (L->phi[s] - L->phi[u]);
+                }
+                next[s] = ph;
+            }
+    memcpy(L->phi, next, sizeof(double) * (size_t)n);
+}
+
 int ora_step(ora_lattice* L, int nsteps) {
     const drp_config* c = &L->c;
     for (int k = 0; k < nsteps; k++) {
@@ -519,7 +576,8 @@ int ora_step(ora_lattice* L, int nsteps) {
         memcpy(L->uo2, L->u2, sizeof(double) * (size_t)L->n);
         memcpy(L->uo3, L->u3, sizeof(double) * (size_t)L->n);
 #pragma omp parallel for num_threads(L->threads) schedule(static)
-        for (int64_t s = 0; s < L->n; s++) moments_at(L, s, 3);
+        for (int64_t s = 0; s < L->n; s++) moments_at(L, s, allen_cahn(L) ? 2 : 3);
+        if (allen_cahn(L)) allen_cahn_phi(L);
         phi_range(L);
         if (les_on(L)) les_tau_update(L);
         L->step = nstep;

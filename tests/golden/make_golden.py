@@ -148,6 +148,38 @@ def case_walls(name, N, steps, D, nu, uw, kappag):
     return data, meta
 
 
+def case_allen_cahn(name, N, steps, D, A, W, start, nu=0.2, uw=0.01, kappag=8.132e-3):
+    """Conservative Allen-Cahn shear droplet, prototype_program/two-phase/dropac.f90
+    (walls normal to z moving from step `start`).  Nx = 3 is hard-coded and the
+    gather to rank 0 names ranks 0..8, so it runs on 9 ranks.  The program writes
+    text only: every 100 steps the plane y = ymax/2 as (x, z, u1, u2, u3, p, phi)
+    with 9 significant digits (output :932-952), and every step
+    (n, shear strain, L, B, D) of its deformation measure to break.d (:929)."""
+    assert steps % 100 == 0
+    params = dict(xmax=str(N - 1), ymax=str(N - 1), zmax=str(N - 1), step=str(steps), start=str(start),
+                  datadir='"./out/"', D=fnum(D) + " * ds", nu1=fnum(nu), nu2=fnum(nu), uw=fnum(uw),
+                  kappag=fnum(kappag), A=fnum(A), W=fnum(W) + " * ds")
+    exe = ref_build.build("prototype_program/two-phase/dropac.f90", f"dropac_{name}", params)
+    keep = list(range(100, steps + 1, 100))
+    data = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        run(exe, 9, tmp)
+        for n in keep:
+            rows = np.loadtxt(os.path.join(tmp, "out", f"{n}.d"))
+            assert rows.shape == (N * N, 7)
+            assert (rows[:, 0].reshape(N, N) == np.arange(N)[None, :]).all()
+            assert (rows[:, 1].reshape(N, N) == np.arange(N)[:, None]).all()
+            data[f"plane_{n}"] = rows[:, 2:].reshape(N, N, 5)   # [z][x][u1, u2, u3, p, phi]
+        data["break"] = np.loadtxt(os.path.join(tmp, "break.d"))  # n, strain, L, B, D per step
+    meta = dict(case=name, program="prototype_program/two-phase/dropac.f90", nranks=9, params=params,
+                phys=dict(nx=N, ny=N, nz=N, nu1=nu, nu2=nu, kappa_g=kappag, ac_a=A, ac_width=W, phi1=-0.5,
+                          phi2=0.5, wall_u=uw, wall_start=start, drop=[0.5 * (N - 1)] * 3 + [D], p_init=0.0,
+                          plane_y=(N - 1) // 2),
+                steps=keep,
+                sha256=ref_build.sha256_of(os.path.join(ref_build.REF_ROOT, "prototype_program/two-phase/dropac.f90")))
+    return data, meta
+
+
 def injected_momentum(N, seed, amp=0.02, noise=0.002):
     """A momentum-population field to inject: equilibria of a Taylor-Green-like
     velocity plus seeded noise (numpy.random.default_rng(seed))."""
@@ -268,6 +300,9 @@ CASES = {
     # deformation measure (:905-944) never exits
     "walls24": lambda: case_walls("walls24", 24, 30000, D=12.0, nu=0.1, uw=0.01, kappag=0.01),
     # Smagorinsky LES (Cs 0.173, Delta 4) with a droplet, injection + Taylor-Green force, 500 steps
+    # conservative Allen-Cahn model (dropac.f90), 24^3, mobility M = (1 - A)/6 = 0.1, W = 4,
+    # walls moving from step 50, 300 steps
+    "ac24": lambda: case_allen_cahn("ac24", 24, 300, D=12.0, A=0.4, W=4.0, start=50),
     # droplet counts of program/percolation.f90 for 71 seeded sphere layouts at 32^3
     "dsd32": lambda: case_dsd("dsd32", 32, seed=31),
     "les16": lambda: case_les("les16", 16, 500, step_input=3, seed=21, delta=4.0, Cs=0.173, Re=80.0, Dv=8.0,

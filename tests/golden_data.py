@@ -7,7 +7,7 @@ import os
 
 import numpy as np
 
-from droplet_amd.abi import DRP_FORCE_TAYLOR_GREEN, DRP_WALL_Z, make_config
+from droplet_amd.abi import DRP_FORCE_TAYLOR_GREEN, DRP_MODEL_ALLEN_CAHN, DRP_WALL_Z, make_config
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -26,7 +26,8 @@ def available(name: str) -> bool:
 def config_for(meta: dict, **over):
     """drp_config reproducing the reference parameter set of a golden case."""
     ph = meta["phys"]
-    kw = dict(nu1=ph["nu1"], nu2=ph["nu2"], kappa_g=ph["kappa_g"], tau_f=ph["tau_f"], kappa_f=ph["kappa_f"])
+    kw = dict(nu1=ph["nu1"], nu2=ph["nu2"], kappa_g=ph["kappa_g"], tau_f=ph.get("tau_f", 0.7),
+              kappa_f=ph.get("kappa_f", 0.01))
     if meta["case"].startswith(("inject", "les")):
         kw.update(force_kind=DRP_FORCE_TAYLOR_GREEN, force_start=meta["step_input"],
                   force_amp=ph["force_amp"], force_k=ph["force_k"])
@@ -35,6 +36,10 @@ def config_for(meta: dict, **over):
                   les_ranks_y=ph["les_ranks_y"], les_ranks_z=ph["les_ranks_z"])
     if "wall_u" in ph:
         kw.update(wall_axis=DRP_WALL_Z, wall_u=ph["wall_u"], wall_start=ph["wall_start"])
+    if "ac_a" in ph:
+        kw = dict(nu1=ph["nu1"], nu2=ph["nu2"], kappa_g=ph["kappa_g"], model=DRP_MODEL_ALLEN_CAHN, ac_a=ph["ac_a"],
+                  ac_width=ph["ac_width"], phi1=ph["phi1"], phi2=ph["phi2"], wall_axis=DRP_WALL_Z,
+                  wall_u=ph["wall_u"], wall_start=ph["wall_start"])
     kw.update(over)
     return make_config(ph["nx"], ph["ny"], ph["nz"], **kw)
 

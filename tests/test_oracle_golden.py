@@ -101,3 +101,29 @@ def test_les_fixture_pins_the_decomposition():
         o.inject_momentum(z["g_injected"])
         o.step(500 - o.step_count)
         assert np.abs(o.fields()["phi"] - z["fields_500"][0]).max() > 1e-5, over
+
+
+def test_allen_cahn_plane_and_deformation():
+    """Conservative Allen-Cahn model (prototype_program/two-phase/dropac.f90) against
+    the fields the compiled program wrote on the plane y = ymax/2 at steps 100, 200
+    and 300 with 9 significant digits (its only field output), and against the
+    deformation D it wrote every step (break.d): the restatement's break_up3 on the
+    same state.  Walls normal to z move from step 50."""
+    z, meta = gd.load("ac24")
+    o = OracleLattice(gd.config_for(meta))
+    o.init_droplets(gd.drop_for(meta), meta["phys"]["p_init"])
+    yp = meta["phys"]["plane_y"]
+    brk = z["break"]
+    for n in meta["steps"]:
+        _run_to(o, n)
+        f = o.fields()
+        ref = z[f"plane_{n}"]   # [z][x][u1, u2, u3, p, phi]
+        for k, fld in enumerate(("u1", "u2", "u3", "p", "phi")):
+            got = f[fld][:, yp, :]
+            want = ref[..., k]
+            # the text output rounds to 9 significant digits (es16.8)
+            np.testing.assert_allclose(got, want, rtol=6e-9, atol=1e-300, err_msg=f"ac24 step {n} {fld}")
+        row = brk[brk[:, 0] == n][0]
+        L, B, D = o.deformation(0)
+        assert abs(L - row[2]) <= 1e-7 * row[2] and abs(B - row[3]) <= 1e-7 * row[3], (n, L, B, row)
+        assert abs(D - row[4]) <= 1e-8, (n, D, row[4])
