@@ -56,6 +56,33 @@ def traffic_from_profiles(kernel_tag: str):
         return None
 
 
+def roofline(kernel_tag: str, bytes_per_lu: int, nodes: int, kernel_ms: float) -> dict:
+    """HBM roofline of one step launch: algorithmic bytes (bytes_per_lu x nodes) over
+    the HIP-event kernel time (achieved, frac), and the HBM bytes rocprofv3 measured
+    per launch (traffic, from profiles/) over the same time (measured_gbs)."""
+    achieved = bytes_per_lu * nodes / (kernel_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(kernel_tag)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "measured_gbs": None if traffic is None else round(traffic / (kernel_ms * 1e-3) / 1e9, 1),
+            "kernel_ms": round(kernel_ms, 5), "bytes_per_lu": bytes_per_lu,
+            "traffic_source": f"profiles/{kernel_tag}.md"}
+
+
+def stated_fp32_error():
+    """The FP32-against-FP64 error measured on the 256^3 shear droplet at Ca 0.1-0.4
+    over 5000 steps (tests/test_gpu_configs.py::test_c2_fp32_error_256), as recorded
+    in profiles/fp32_error_shear256.json."""
+    path = os.path.join(ROOT, "profiles", "fp32_error_shear256.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except Exception:
+        return None
+
+
 def run_gpu(cfg, drops, steps, warmup, shape=None):
     from droplet_amd.lattice import Lattice
     lat = Lattice(cfg)
@@ -134,7 +161,6 @@ def main():
     lat, wall, ms = run_gpu(cfg, drops, args.steps, args.warmup)
     mlups = nodes * args.steps / wall / 1e6
     kernel_ms = ms / args.steps
-    achieved = BYTES_PER_LU["f64"] * nodes / (kernel_ms * 1e-3) / 1e9
     stats = lat.reduce()
     tile = lat.tile_shape
 
@@ -146,7 +172,6 @@ def main():
         cfg32, _, _ = shear_config(N, precision=1)
         lat32, wall32, ms32 = run_gpu(cfg32, drops, args.steps, args.warmup)
         k32 = ms32 / args.steps
-        ach32 = BYTES_PER_LU["f32"] * nodes / (k32 * 1e-3) / 1e9
         phi64, phi32 = lat.macro(0), lat32.macro(0)
         u64, u32 = lat.macro(2), lat32.macro(2)
         D64, D32 = lat.deformation(2)[2], lat32.deformation(2)[2]
@@ -154,10 +179,11 @@ def main():
         mlups32 = nodes * args.steps / wall32 / 1e6
         fp32 = {"value": round(mlups32, 2), "unit": "MLUPS", "dtype": "f32 (shifted storage)",
                 "ms_per_step": round(wall32 * 1e3 / args.steps, 4), "speedup_vs_fp64": round(mlups32 / mlups, 3),
-                "roofline": {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(f"step_fp32_{N}"),
-                             "kernel_ms": round(k32, 5), "bytes_per_lu": BYTES_PER_LU["f32"]},
-                "error_vs_fp64": {"after_steps": args.warmup + args.steps,
+                "roofline": roofline(f"step_fp32_{N}", BYTES_PER_LU["f32"], nodes, k32),
+                "stated_error": stated_fp32_error(),
+                # difference from the FP64 run of this bench after its few steps (drop not
+                # yet deformed); the stated error above is taken at deformed states
+                "drift_after_bench_steps": {"after_steps": args.warmup + args.steps,
                                   "phi_linf_rel": float(np.abs(phi32 - phi64).max() / cfg.phi2),
                                   "u1_linf": float(np.abs(u32 - u64).max()),
                                   "deformation_abs": abs(D32 - D64), "deformation_fp64": D64,
@@ -195,10 +221,7 @@ def main():
                                f"Ca {phys['Ca']}, FP64 populations",
                    "grid": [N, N, N], "wall_u": round(phys["U"], 6), "kappa_g": phys["kappa_g"],
                    "tau_f": 0.7, "nu": 0.1, "tile_shape": tile},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"step_fp64_{N}"),
-                     "kernel_ms": round(kernel_ms, 5), "bytes_per_lu": BYTES_PER_LU["f64"]},
+        "roofline": roofline(f"step_fp64_{N}", BYTES_PER_LU["f64"], nodes, kernel_ms),
         "cpu_baseline": cpu,
         "fp32_shifted": fp32,
         "phi_mass": stats["phi_mass"],

@@ -1,0 +1,238 @@
+"""The workloads the path is specified on, run at their stated sizes on the GPU.
+
+* shear droplet 64^3 (radius 12, walls normal to y moving -+0.01 along x from
+  step 1, tau_f = 0.7, nu = 0.1, kappa_f = 0.01), 2000 FP64 steps, against the
+  CPU restatement bit for bit, with the Taylor deformation every 100 steps
+  (prototype_program/two-phase/dropch.f90:666-698 walls, :857-948 break_up3);
+  the same from a diffuse tanh interface (width 2);
+* shear droplet 256^3 (radius 40), Ca = 0.1..0.4, 5000 steps, FP32 shifted
+  storage against FP64: deformation D(t) every 100 steps within 1e-4, phi
+  L-infinity, phi-mass drift and the break-up step (first census count > 1,
+  program/percolation.f90);
+* ensemble of 64 shear-droplet members of 48..128^3 (numpy.random.default_rng(4)),
+  one autotuned batched launch per step, 3000 FP32 steps against the same batch
+  in FP64; and a small FP64 batch against the restatement bit for bit.
+
+When DROPLET_RESULTS_DIR is set, each test writes the numbers it measured there
+as JSON.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from droplet_amd.abi import DRP_FP32_SHIFTED, DRP_FP64, DRP_WALL_Y, make_config
+from oracle.pyoracle import OracleLattice
+
+pytestmark = pytest.mark.gpu
+FIELDS = ("phi", "p", "u1", "u2", "u3")
+KAPPA_G = 0.002
+SIGMA = 1.7039 * KAPPA_G ** 0.9991   # sigma = 1.7039 kappa_g^0.9991, program/lbm_drop_xy.f90:51
+
+
+def _record(name, data):
+    d = os.environ.get("DROPLET_RESULTS_DIR")
+    if d:
+        os.makedirs(d, exist_ok=True)
+        with open(os.path.join(d, f"{name}.json"), "w") as fh:
+            json.dump(data, fh, indent=1)
+
+
+def _threads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def _shear(N, U, precision=DRP_FP64, wall_start=1):
+    return make_config(N, precision=precision, nu1=0.1, nu2=0.1, tau_f=0.7, kappa_f=0.01, kappa_g=KAPPA_G,
+                       wall_axis=DRP_WALL_Y, wall_u=U, wall_start=wall_start)
+
+
+def _wall_speed(N, D, Ca):
+    # Ca = U nu D / (H sigma) with H the wall distance (dropch.f90:64-65)
+    return Ca * (N - 1) * SIGMA / (0.1 * D)
+
+
+def _assert_same(gpu, ora, label):
+    a, b = gpu.fields(), ora.fields()
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), f"{label}: {k} differs (max {np.abs(a[k] - b[k]).max():.3e})"
+
+
+def _run_lockstep(lat, ora, steps, check_at, label):
+    """Step both to `steps`, bit-compare the fields at check_at, compare D (modes 0
+    and 2) every 100 steps; returns the D(t) table."""
+    table = []
+    marks = sorted(set(check_at) | set(range(100, steps + 1, 100)))
+    for n in marks:
+        lat.step(n - lat.step_count)
+        ora.step(n - ora.step_count)
+        if n in check_at:
+            _assert_same(lat, ora, f"{label}@{n}")
+        if n % 100 == 0:
+            row = [n]
+            for mode in (0, 2):
+                dg, do = lat.deformation(mode)[2], ora.deformation(mode)[2]
+                assert abs(dg - do) <= 1e-12, f"{label}@{n}: D mode {mode} {dg} vs {do}"
+                row.append(dg)
+            table.append(row)
+    return table
+
+
+def test_c1_shear_droplet_64():
+    """64^3, radius 12 at the centre, 2000 FP64 steps: bit-identical to the
+    restatement at steps 1, 100 and 2000, D(t) equal every 100 steps."""
+    N = 64
+    cfg = _shear(N, 0.01)
+    drop = [(0.5 * (N - 1),) * 3 + (24.0,)]
+    from droplet_amd.lattice import Lattice
+    lat = Lattice(cfg)
+    ora = OracleLattice(cfg, threads=_threads())
+    lat.init_droplets(drop)
+    ora.init_droplets(drop)
+    table = _run_lockstep(lat, ora, 2000, (1, 100, 2000), "C1")
+    st = lat.reduce()
+    assert table[-1][2] > 0.0   # sheared drop elongates
+    _record("c1_shear64", {"steps": 2000, "D_mode0_mode2": table, "phi_mass": st["phi_mass"]})
+
+
+def test_c1_tanh_profile():
+    """The same case from phi = mid + half tanh((R - r)/(sqrt2 W)), W = 2: the
+    initial field against the formula, then 300 steps bit-identical to the
+    restatement started from the same fields."""
+    from droplet_amd.lattice import Lattice
+    N, W, R = 64, 2.0, 12.0
+    cfg = _shear(N, 0.01)
+    c = 0.5 * (N - 1)
+    lat = Lattice(cfg)
+    lat.init_droplets_tanh([(c, c, c, 2 * R)], W)
+    f0 = lat.fields()
+    z, y, x = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    r = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+    want = 0.5 * (cfg.phi1 + cfg.phi2) + 0.5 * (cfg.phi2 - cfg.phi1) * np.tanh((R - r) / (math.sqrt(2) * W))
+    assert np.abs(f0["phi"] - want).max() <= 1e-14 * cfg.phi2
+    assert np.all(f0["p"] == 1.0 / 3.0) and not np.any(f0["u1"])
+    ora = OracleLattice(cfg, threads=_threads())
+    ora.init_fields(*[f0[k] for k in FIELDS])
+    table = _run_lockstep(lat, ora, 300, (1, 300), "C1 tanh")
+    _record("c1_tanh64", {"steps": 300, "D_mode0_mode2": table})
+
+
+def _breakup(lat):
+    return len(lat.droplet_census()[0]) > 1
+
+
+@pytest.mark.parametrize("Ca", [0.1, 0.2, 0.3, 0.4])
+def test_c2_fp32_error_256(Ca):
+    """256^3, radius 40, 5000 steps: FP32 shifted storage against FP64 on the same
+    initial state.  D (second moments, smooth in phi) within 1e-4 absolute at
+    every 100th step; phi relative L-infinity and phi-mass drift recorded; the
+    break-up step (first step with more than one droplet) equal in both forms."""
+    from droplet_amd.lattice import Lattice
+    N, D = 256, 80.0
+    U = _wall_speed(N, D, Ca)
+    drop = [(0.5 * (N - 1),) * 3 + (D,)]
+    l64, l32 = Lattice(_shear(N, U)), Lattice(_shear(N, U, DRP_FP32_SHIFTED))
+    l64.init_droplets(drop)
+    l32.init_droplets(drop)
+    m0 = l64.reduce()["phi_mass"]
+    rows, worst, brk = [], 0.0, {"fp64": None, "fp32": None}
+    for n in range(100, 5001, 100):
+        l64.step(100)
+        l32.step(100)
+        d64, d32 = l64.deformation(2)[2], l32.deformation(2)[2]
+        worst = max(worst, abs(d32 - d64))
+        rows.append([n, d64, d32])
+        for k, lat in (("fp64", l64), ("fp32", l32)):
+            if brk[k] is None and _breakup(lat):
+                brk[k] = n
+    phi64, phi32 = l64.macro(0), l32.macro(0)
+    m64, m32 = l64.reduce()["phi_mass"], l32.reduce()["phi_mass"]
+    res = {"Ca": Ca, "U": U, "steps": 5000, "D_fp64_final": rows[-1][1], "D_abs_err_max": worst,
+           "phi_linf_rel": float(np.abs(phi32 - phi64).max() / l64.cfg.phi2),
+           "u1_linf": float(np.abs(l32.macro(2) - l64.macro(2)).max()),
+           "mass_drift_fp64": abs(m64 - m0) / m0, "mass_drift_fp32": abs(m32 - m0) / m0,
+           "breakup_step": brk, "D_t": rows}
+    _record(f"c2_fp32_error_256_Ca{Ca}", res)
+    print(json.dumps({k: v for k, v in res.items() if k != "D_t"}))
+    assert worst <= 1e-4, f"Ca {Ca}: FP32 deformation off by {worst:.3e}"
+    assert res["mass_drift_fp64"] <= 1e-12
+    assert brk["fp32"] == brk["fp64"]
+
+
+def _ensemble(precision, members=64):
+    """default_rng(4): N uniform in {48, 64, 96, 128}, radius N/5, Ca uniform in
+    [0.05, 0.5], centre offsets uniform in [-2, 2] cells."""
+    rng = np.random.default_rng(4)
+    cfgs, drops = [], []
+    for _ in range(members):
+        N = int(rng.choice([48, 64, 96, 128]))
+        Ca = float(rng.uniform(0.05, 0.5))
+        D = 2.0 * N / 5.0
+        cfgs.append(_shear(N, _wall_speed(N, D, Ca), precision))
+        off = rng.uniform(-2.0, 2.0, 3)
+        drops.append([tuple(0.5 * (N - 1) + off) + (D,)])
+    return cfgs, drops
+
+
+def test_c4_ensemble_64_members():
+    """64 members, one autotuned batched launch per step (hipGraph cycles), 3000
+    FP32 steps; the same batch in FP64 as the yardstick: per-member D within 1e-4,
+    FP64 phi mass conserved, FP32 phi-mass drift recorded."""
+    from droplet_amd.lattice import Batch
+    steps = 3000
+    out = {}
+    for prec, tag in ((DRP_FP32_SHIFTED, "fp32"), (DRP_FP64, "fp64")):
+        cfgs, drops = _ensemble(prec)
+        b = Batch(cfgs, tile_shape="auto")
+        for m, d in zip(b.members, drops):
+            m.init_droplets(d)
+        m0 = [m.reduce()["phi_mass"] for m in b.members]
+        ms = b.time_steps(steps)
+        nodes = sum(c.nx * c.ny * c.nz for c in cfgs)
+        out[tag] = {"shape": b.tile_shape_used, "tuned_ms": b.tuned, "ms_per_step": ms / steps,
+                    "MLUPS": nodes * steps / (ms * 1e-3) / 1e6, "nodes": nodes,
+                    "D": [m.deformation(2)[2] for m in b.members],
+                    "mass_drift": [abs(m.reduce()["phi_mass"] - a) / a for m, a in zip(b.members, m0)]}
+        assert all(m.step_count == steps for m in b.members)
+        b.close()
+    dD = np.abs(np.array(out["fp32"]["D"]) - np.array(out["fp64"]["D"]))
+    out["D_abs_err_max"] = float(dD.max())
+    _record("c4_ensemble", out)
+    print(json.dumps({k: (v if not isinstance(v, dict) else {kk: vv for kk, vv in v.items() if kk not in ("D", "mass_drift")})
+                      for k, v in out.items()}))
+    assert max(out["fp64"]["mass_drift"]) <= 1e-12
+    assert max(out["fp32"]["mass_drift"]) <= 1e-5
+    assert np.all(np.array(out["fp64"]["D"]) > 0.0)
+    assert dD.max() <= 1e-4
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_fp64_members_vs_oracle(graph):
+    """Three FP64 members of different sizes, wall speeds and wall start steps
+    (nu(phi) on), batched with and without the captured six-step graph: every
+    member bit-identical to the restatement after 20 and 26 steps."""
+    from droplet_amd.lattice import Batch
+    specs = [((16, 16, 16), 0.02, 1), ((20, 18, 22), 0.01, 4), ((24, 20, 16), 0.03, 9)]
+    cfgs, drops = [], []
+    for (nx, ny, nz), U, ws in specs:
+        cfgs.append(make_config(nx, ny, nz, nu1=0.1, nu2=0.2, kappa_g=0.01, wall_axis=DRP_WALL_Y, wall_u=U,
+                                wall_start=ws))
+        drops.append([(0.5 * (nx - 1), 0.5 * (ny - 1) + 0.5, 0.5 * (nz - 1), 0.45 * min(nx, ny, nz))])
+    b = Batch(cfgs)
+    b.use_graph(graph)
+    oras = [OracleLattice(c, threads=2) for c in cfgs]
+    for m, o, d in zip(b.members, oras, drops):
+        m.init_droplets(d)
+        o.init_droplets(d)
+    b.step(3)
+    b.step(17)   # crosses every wall start step, then whole graph cycles
+    for o in oras:
+        o.step(20)
+    for i, (m, o) in enumerate(zip(b.members, oras)):
+        _assert_same(m, o, f"batch member {i}@20")
+    b.step(6)
+    for i, (m, o) in enumerate(zip(b.members, oras)):
+        o.step(6)
+        _assert_same(m, o, f"batch member {i}@26")
