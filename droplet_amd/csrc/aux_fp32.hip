@@ -134,6 +134,71 @@ __global__ void les_tau_fp32(int nx, int ny, int nz, Phys P, const float* __rest
     tau[s] = (float)(0.5 + 3.0 * (nu + P.les_c * sqrt(2.0 * (double)sum)));
 }
 
+// The same tau with every neighbour current, marching in z over a 64 x 4 column
+// of nodes with the planes z-1, z, z+1 of u (one-node x-y halo) in a ring of
+// three LDS planes; the arithmetic and its order are those of les_tau_fp32.
+constexpr int ZM32_TX = 64, ZM32_TY = 4, ZM32_ZC = 16;
+__device__ __forceinline__ int wrap_zm32(int v, int n, bool small) {
+    if (small) return ((v % n) + n) % n;
+    return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+__global__ __launch_bounds__(ZM32_TX* ZM32_TY) void les_tau_fp32_zm(int nx, int ny, int nz, Phys P,
+                                                                     const float* __restrict__ mac,
+                                                                     const uint64_t* __restrict__ range, int slot,
+                                                                     float* __restrict__ tau, int tiles_x, int tiles_y) {
+    constexpr int HX = ZM32_TX + 2, HP = HX * (ZM32_TY + 2);
+    __shared__ float su[3][3][HP];  // [ring slot][component][plane node]
+    const int64_t N = (int64_t)nx * ny * nz;
+    const int x0 = (blockIdx.x % tiles_x) * ZM32_TX, y0 = ((blockIdx.x / tiles_x) % tiles_y) * ZM32_TY;
+    const int zs = (blockIdx.x / (tiles_x * tiles_y)) * ZM32_ZC, ze = min(zs + ZM32_ZC, nz);
+    const bool small = nx <= ZM32_TX || ny <= ZM32_TY || nz <= 2;
+    const int tx = threadIdx.x % ZM32_TX, ty = threadIdx.x / ZM32_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool in = x < nx && y < ny;
+    const double rmax = from_ordered_bits(range[2 * slot]), rmin = from_ordered_bits(range[2 * slot + 1]);
+    const double rg = rmax - rmin;
+    auto load = [&](int zg, float (*dst)[HP]) {
+        const int64_t zoff = (int64_t)nx * ny * wrap_zm32(zg, nz, small);
+        for (int h = threadIdx.x; h < HP; h += ZM32_TX * ZM32_TY) {
+            const int64_t g = wrap_zm32(x0 + h % HX - 1, nx, small) + (int64_t)nx * wrap_zm32(y0 + h / HX - 1, ny, small) + zoff;
+            for (int k = 0; k < 3; k++) dst[k][h] = mac[(2 + k) * N + g];
+        }
+    };
+    load(zs - 1, su[0]);
+    load(zs, su[1]);
+    for (int z = zs; z < ze; z++) {
+        load(z + 1, su[(z - zs + 2) % 3]);
+        __syncthreads();
+        if (in) {
+            const int rz[3] = {(z - zs) % 3, (z - zs + 1) % 3, (z - zs + 2) % 3};  // planes z-1, z, z+1
+            float gu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            sforx<1, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                const int h = (tx + 1 + D::cx) + HX * (ty + 1 + D::cy);
+                for (int k = 0; k < 3; k++) {
+                    const float v = su[rz[1 + D::cz]][k][h];
+                    gu[k][0] = acc<D::cx>(gu[k][0], v);
+                    gu[k][1] = acc<D::cy>(gu[k][1], v);
+                    gu[k][2] = acc<D::cz>(gu[k][2], v);
+                }
+            });
+            float sum = 0.0f;
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    const float sab = 0.05f * (gu[a][b] + gu[b][a]);
+                    sum += sab * sab;
+                }
+            const int64_t s = x + (int64_t)nx * (y + (int64_t)ny * z);
+            const double phi = P.phi1 + (double)mac[s];
+            const double frac = rg == 0.0 ? 0.0 : (phi - rmin) / rg;
+            const double nu = frac * P.dnu + P.nu1;
+            tau[s] = (float)(0.5 + 3.0 * (nu + P.les_c * sqrt(2.0 * (double)sum)));
+        }
+        __syncthreads();
+    }
+}
+
 hipError_t launch_to_shifted(int64_t n, const ShiftRef& r, const double* pop64, const double* mac64, float* pop32,
                              float* mac32, hipStream_t st) {
     hipLaunchKernelGGL(to_shifted, grid_for(n), dim3(256), 0, st, n, r, pop64, mac64, pop32, mac32);
@@ -160,8 +225,14 @@ hipError_t launch_moments_fp32(int64_t n, const float* f, const float* g, float*
 hipError_t launch_les_tau_fp32(int nx, int ny, int nz, const Phys& ph, const float* mac, const float* mac_old,
                                const LesBlocks& blk, const uint64_t* range, int slot, float* tau, hipStream_t st) {
     const int64_t n = (int64_t)nx * ny * nz;
-    hipLaunchKernelGGL(les_tau_fp32, grid_for(n), dim3(256), 0, st, nx, ny, nz, ph, mac, mac_old, blk, range, slot,
-                       tau);
+    if (blk.bx > 0 && mac_old != mac) {
+        hipLaunchKernelGGL(les_tau_fp32, grid_for(n), dim3(256), 0, st, nx, ny, nz, ph, mac, mac_old, blk, range, slot,
+                           tau);
+    } else {
+        const int mx = (nx + ZM32_TX - 1) / ZM32_TX, my = (ny + ZM32_TY - 1) / ZM32_TY, mz = (nz + ZM32_ZC - 1) / ZM32_ZC;
+        hipLaunchKernelGGL(les_tau_fp32_zm, dim3((unsigned)(mx * my * mz)), dim3(ZM32_TX * ZM32_TY), 0, st, nx, ny, nz,
+                           ph, mac, range, slot, tau, mx, my);
+    }
     return hipGetLastError();
 }
 

@@ -205,6 +205,80 @@ __global__ __launch_bounds__(LES_TX* LES_TY* LES_TZ) void les_tau_fp64(Grid G, P
     tau[s] = 0.5 + 3.0 * (nu + nue);
 }
 
+// The same tau with every neighbour current (no rank blocks), marching in z: a
+// workgroup owns a ZM_TX x ZM_TY column of nodes over ZM_ZC planes and keeps the
+// planes z-1, z, z+1 of u (with a one-node x-y halo) in a ring of three LDS
+// planes, loading one new plane per step in z.  u is read from L2 / HBM about
+// (66 x 6)/(64 x 4) (ZM_ZC + 2)/ZM_ZC times instead of the 3.2 times of
+// 32 x 4 x 2 tiles.  Sums are formed in the order of les_tau_fp64, so both give
+// identical tau.
+constexpr int ZM_TX = 64, ZM_TY = 4, ZM_ZC = 16;
+template <typename T>
+__device__ __forceinline__ void zm_load_plane(const Grid& G, const T* __restrict__ mac, int64_t N, int x0, int y0,
+                                              int zg, bool small, T (*dst)[(ZM_TX + 2) * (ZM_TY + 2)]) {
+    constexpr int HX = ZM_TX + 2, HP = HX * (ZM_TY + 2);
+    const int64_t zoff = (int64_t)G.nx * G.ny * wrap_any(zg, G.nz, small);
+    for (int h = threadIdx.x; h < HP; h += ZM_TX * ZM_TY) {
+        const int64_t g = wrap_any(x0 + h % HX - 1, G.nx, small) + (int64_t)G.nx * wrap_any(y0 + h / HX - 1, G.ny, small) + zoff;
+        for (int k = 0; k < 3; k++) dst[k][h] = mac[(2 + k) * N + g];
+    }
+}
+
+__global__ __launch_bounds__(ZM_TX* ZM_TY) void les_tau_fp64_zm(Grid G, Phys P, const double* __restrict__ mac,
+                                                                 const uint64_t* __restrict__ range, int slot,
+                                                                 double* __restrict__ tau, int tiles_x, int tiles_y) {
+    constexpr int HX = ZM_TX + 2, HP = HX * (ZM_TY + 2);
+    __shared__ double su[3][3][HP];  // [ring slot][component][plane node]
+    const int64_t N = (int64_t)G.nx * G.ny * G.nz;
+    const int x0 = (blockIdx.x % tiles_x) * ZM_TX, y0 = ((blockIdx.x / tiles_x) % tiles_y) * ZM_TY;
+    const int zs = (blockIdx.x / (tiles_x * tiles_y)) * ZM_ZC, ze = min(zs + ZM_ZC, G.nz);
+    const bool small = G.nx <= ZM_TX || G.ny <= ZM_TY || G.nz <= 2;
+    const int tx = threadIdx.x % ZM_TX, ty = threadIdx.x / ZM_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool in = x < G.nx && y < G.ny;
+    const double rmax = from_ordered_bits(range[2 * slot]), rmin = from_ordered_bits(range[2 * slot + 1]);
+    const double rg = rmax - rmin;
+    // ring slot of plane z' is (z' - zs + 1) mod 3
+    zm_load_plane<double>(G, mac, N, x0, y0, zs - 1, small, su[0]);
+    zm_load_plane<double>(G, mac, N, x0, y0, zs, small, su[1]);
+    for (int z = zs; z < ze; z++) {
+        zm_load_plane<double>(G, mac, N, x0, y0, z + 1, small, su[(z - zs + 2) % 3]);
+        __syncthreads();
+        if (in) {
+            const int rz[3] = {(z - zs) % 3, (z - zs + 1) % 3, (z - zs + 2) % 3};  // planes z-1, z, z+1
+            double gu[3][3];
+            for (int k = 0; k < 3; k++)
+                for (int a = 0; a < 3; a++) gu[k][a] = 0.0;
+            sfor<1, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                const int h = (tx + 1 + D::cx) + HX * (ty + 1 + D::cy);
+                for (int k = 0; k < 3; k++) {
+                    const double v = su[rz[1 + D::cz]][k][h];
+                    gu[k][0] = acc<D::cx>(gu[k][0], v);
+                    gu[k][1] = acc<D::cy>(gu[k][1], v);
+                    gu[k][2] = acc<D::cz>(gu[k][2], v);
+                }
+            });
+            for (int k = 0; k < 3; k++)
+                for (int a = 0; a < 3; a++) gu[k][a] = gu[k][a] / 10.0;
+            double sum = 0.0;
+            for (int b = 0; b < 3; b++)
+                for (int a = 0; a < 3; a++) {
+                    const double sab = 0.5 * (gu[a][b] + gu[b][a]);
+                    sum = sum + sab * sab;
+                }
+            const double nue = P.les_c * sqrt(2.0 * sum);
+            const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
+            double frac = (mac[s] - rmin) / rg;
+            if (rg == 0.0) frac = 0.0;
+            const double nu = frac * P.dnu + P.nu1;
+            tau[s] = 0.5 + 3.0 * (nu + nue);
+        }
+        __syncthreads();
+    }
+}
+
 // Diagnostics: per-workgroup partial sums, reduced in fixed order on the host.
 // [0] sum phi  [1] sum 0.5|u|^2  [2] sum (d_a u_b + d_b u_a)^2  [3] sum |grad phi|
 // [4] sum psi(phi)  [5] sum p  [6..8] sum u  [9] sum_ab d_a phi d_b phi S_ab
@@ -286,10 +360,13 @@ hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const dou
     Grid G{nx, ny, nz};
     const int tx = (nx + LES_TX - 1) / LES_TX, ty = (ny + LES_TY - 1) / LES_TY, tz = (nz + LES_TZ - 1) / LES_TZ;
     const dim3 grid((unsigned)(tx * ty * tz)), block(LES_TX * LES_TY * LES_TZ);
-    if (blk.bx > 0 && mac_old != mac)
+    if (blk.bx > 0 && mac_old != mac) {
         hipLaunchKernelGGL(les_tau_fp64<true>, grid, block, 0, st, G, ph, mac, mac_old, blk, range, slot, tau, tx, ty);
-    else
-        hipLaunchKernelGGL(les_tau_fp64<false>, grid, block, 0, st, G, ph, mac, mac_old, blk, range, slot, tau, tx, ty);
+    } else {
+        const int mx = (nx + ZM_TX - 1) / ZM_TX, my = (ny + ZM_TY - 1) / ZM_TY, mz = (nz + ZM_ZC - 1) / ZM_ZC;
+        hipLaunchKernelGGL(les_tau_fp64_zm, dim3((unsigned)(mx * my * mz)), dim3(ZM_TX * ZM_TY), 0, st, G, ph, mac, range,
+                           slot, tau, mx, my);
+    }
     return hipGetLastError();
 }
 
