@@ -29,6 +29,8 @@ DRP_FORCE_FIELD = 2
 DRP_FORCE_SPECTRAL = 3
 DRP_MODEL_FREE_ENERGY = 0
 DRP_MODEL_ALLEN_CAHN = 1
+DRP_LES_STRAIN = 0
+DRP_LES_NONEQ = 1
 DRP_FILE_FIELDS, DRP_FILE_FG, DRP_FILE_G, DRP_FILE_FG_G = 0, 1, 2, 3
 DRP_FILE_LAYOUT_XZ = 0x100
 
@@ -59,7 +61,7 @@ class Config(C.Structure):
         ("force_amp", C.c_double), ("force_k", C.c_double),
         ("les_cs", C.c_double), ("les_delta", C.c_double),
         ("les_ranks_x", C.c_int32), ("les_ranks_y", C.c_int32), ("les_ranks_z", C.c_int32),
-        ("model", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("model", C.c_int32), ("les_model", C.c_int32), ("reserved", C.c_int32 * 3),
         ("ac_a", C.c_double), ("ac_width", C.c_double),
     ]
 

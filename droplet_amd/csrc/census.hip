@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "../../include/droplet.h"
@@ -170,5 +171,40 @@ extern "C" int drp_droplet_census(drp_lattice* L, double threshold, int64_t* cou
         first[k] = c[k].first;
         volumes[k] = c[k].second;
     }
+    return DRP_OK;
+}
+
+// Droplet size distribution of the current state (program/mkdsd.f90 over the
+// radii program/percolation.f90 derives from cluster volumes, :298):
+// r = (3 V / (4 * 3.14))^(1/3) (the reference's 3.14), d = 2 r / scale; droplets
+// with d > d_max are dropped (mkdsd.f90 :68-72); counts[i] = number with
+// int(d / bin) == i for i < nbins (d > 0).  *total receives the number counted.
+extern "C" int drp_droplet_size_histogram(drp_lattice* L, double threshold, double scale, double bin, double d_max,
+                                          int64_t* counts, int64_t nbins, int64_t* total) {
+    if (!L || !counts || !total || nbins < 1 || !(scale > 0.0) || !(bin > 0.0))
+        return drp_fail(DRP_ERR_ARG, "bad histogram arguments");
+    int64_t nc = 0, cap = 1 << 16;
+    std::vector<int64_t> vol((size_t)cap), first((size_t)cap);
+    int rc = drp_droplet_census(L, threshold, &nc, vol.data(), first.data(), cap, nullptr, 0);
+    if (!rc && nc > cap) {  // more droplets than the first guess: once more with room for all
+        cap = nc;
+        vol.resize((size_t)cap);
+        first.resize((size_t)cap);
+        rc = drp_droplet_census(L, threshold, &nc, vol.data(), first.data(), cap, nullptr, 0);
+    }
+    if (rc) return rc;
+    for (int64_t i = 0; i < nbins; i++) counts[i] = 0;
+    int64_t t = 0;
+    for (int64_t k = 0; k < nc; k++) {
+        const double r = std::pow(3.0 * (double)vol[k] / (4.0 * 3.14), 1.0 / 3.0);
+        const double d = 2.0 * r / scale;
+        if (d > d_max || !(d > 0.0)) continue;
+        const int64_t i = (int64_t)(d / bin);
+        if (i < nbins) {
+            counts[i]++;
+            t++;
+        }
+    }
+    *total = t;
     return DRP_OK;
 }

@@ -65,6 +65,11 @@ struct StepArgs {
     int wrap_mod;         // set by the launcher: a grid extent does not exceed the tile extent
     int model;            // 0 free energy (phi populations), 1 Allen-Cahn (phi stencil update)
     double ac_a, ac_m, ac_w;  // Allen-Cahn A, mobility M = (1 - A)/6, interface width W
+    // non-equilibrium Smagorinsky: sqrt(Pi:Pi) per node from the last step (in) and
+    // of this step (out); 18 Cs^2 D^2, 6 Cs^2 D^2, Cs^2 D^2 (left to right)
+    const double* qh_in;
+    double* qh_out;
+    double neq_c18, neq_c6, neq_cc;
 };
 
 hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st, const BatchLaunch* batch = nullptr);

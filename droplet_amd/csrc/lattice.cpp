@@ -110,6 +110,12 @@ static int validate(const drp_config* c) {
         return drp_fail(DRP_ERR_UNSUPPORTED, "anu != 0 (velocity-gradient term of geq) is not implemented");
     if (c->model != DRP_MODEL_FREE_ENERGY && c->model != DRP_MODEL_ALLEN_CAHN)
         return drp_fail(DRP_ERR_ARG, "unknown model %d", c->model);
+    if (c->les_model != DRP_LES_STRAIN && c->les_model != DRP_LES_NONEQ)
+        return drp_fail(DRP_ERR_ARG, "unknown les_model %d", c->les_model);
+    if (c->les_model == DRP_LES_NONEQ && c->les_cs != 0.0 && c->les_delta != 0.0 &&
+        (c->precision != DRP_FP64 || c->les_ranks_x != 0 || c->model != DRP_MODEL_FREE_ENERGY))
+        return drp_fail(DRP_ERR_UNSUPPORTED, "the non-equilibrium Smagorinsky model runs in FP64, free-energy "
+                                             "model, without rank blocks");
     if (c->model == DRP_MODEL_ALLEN_CAHN) {
         if (!(c->ac_width > 0.0)) return drp_fail(DRP_ERR_ARG, "Allen-Cahn interface width ac_width must be positive");
         if (!(c->ac_a >= 0.0 && c->ac_a < 1.0)) return drp_fail(DRP_ERR_ARG, "Allen-Cahn ac_a must lie in [0, 1)");
@@ -195,7 +201,9 @@ int lattice_alloc(drp_lattice* L) {
         rc |= dalloc((void**)&L->mac[k], 5 * (size_t)n * esz);
     }
     if (!rc) rc = dalloc((void**)&L->range, 6 * sizeof(uint64_t));
-    if (!rc && L->phys.les) rc = dalloc((void**)&L->tau, (size_t)n * sizeof(double));
+    const bool noneq = L->phys.les && L->cfg.les_model == DRP_LES_NONEQ;
+    if (!rc && L->phys.les && !noneq) rc = dalloc((void**)&L->tau, (size_t)n * sizeof(double));
+    for (int k = 0; k < 2 && !rc && noneq; k++) rc = dalloc((void**)&L->qh[k], (size_t)n * sizeof(double));
     if (!rc && (L->cfg.force_kind == DRP_FORCE_FIELD || L->cfg.force_kind == DRP_FORCE_SPECTRAL)) {
         rc = dalloc((void**)&L->force, 3 * (size_t)n * sizeof(double));
         if (!rc) HIP_TRY(hipMemsetAsync(L->force, 0, 3 * (size_t)n * sizeof(double), L->stream));
@@ -248,6 +256,8 @@ void lattice_free(drp_lattice* L) {
     }
     if (L->range) (void)hipFree(L->range);
     if (L->tau) (void)hipFree(L->tau);
+    for (int k = 0; k < 2; k++)
+        if (L->qh[k]) (void)hipFree(L->qh[k]);
     if (L->force) (void)hipFree(L->force);
     if (L->spec) (void)hipFree(L->spec);
     if (L->tg) (void)hipFree(L->tg);
@@ -279,6 +289,13 @@ int reset_range(drp_lattice* L) {
 int les_update_tau(drp_lattice* L, bool fresh) {
     if (!L->phys.les) return DRP_OK;
     const drp_config& c = L->cfg;
+    if (c.les_model == DRP_LES_NONEQ) {
+        // the step kernel keeps sqrt(Pi:Pi) itself; a state that replaced the
+        // lattice's is an equilibrium or a restart: no non-equilibrium flux yet
+        if (fresh)
+            for (int k = 0; k < 2; k++) HIP_TRY(hipMemsetAsync(L->qh[k], 0, (size_t)L->n * sizeof(double), L->stream));
+        return DRP_OK;
+    }
     LesBlocks B{0, 0, 0};
     if (!fresh && c.les_ranks_x > 0) B = LesBlocks{c.nx / c.les_ranks_x, c.ny / c.les_ranks_y, c.nz / c.les_ranks_z};
     const int slot = (int)(L->step % 3);
@@ -390,7 +407,15 @@ StepArgs make_step_args(drp_lattice* L, int64_t nstep) {
     a.g_out = fo + 15 * n;
     a.mac_in = (const double*)L->mac[L->cur];
     a.mac_out = (double*)L->mac[1 - L->cur];
-    a.tau_field = P.les ? L->tau : nullptr;
+    a.tau_field = P.les ? L->tau : nullptr;  // null for the non-equilibrium model (no tau field)
+    if (P.les && c.les_model == DRP_LES_NONEQ) {
+        const double cs = c.les_cs, dl = c.les_delta;
+        a.qh_in = L->qh[L->cur];
+        a.qh_out = L->qh[1 - L->cur];
+        a.neq_c18 = 18.0 * cs * cs * dl * dl;
+        a.neq_c6 = 6.0 * cs * cs * dl * dl;
+        a.neq_cc = cs * cs * dl * dl;
+    }
     a.tau_f = P.tau_f;
     a.kf = P.kf;
     a.kf6 = P.kf6;

@@ -24,6 +24,7 @@ struct drp_lattice {
     void* mac[2] = {nullptr, nullptr};  // phi, p, u1, u2, u3 (5n)
     uint64_t* range = nullptr;          // 3 rotating (max, min) slots of the phi range
     double* tau = nullptr;              // LES relaxation-time field
+    double* qh[2] = {nullptr, nullptr}; // non-equilibrium LES: sqrt(Pi:Pi) per node, with the populations' parity
     double* force = nullptr;            // FORCE_FIELD / SPECTRAL: fx | fy | fz
     double* spec = nullptr;             // SPECTRAL: transform work space and twiddle tables
     drp::SpectralWork specw{};

@@ -61,7 +61,10 @@ __device__ __forceinline__ double div_const(double x, double c, double r) {
     return __builtin_fma(e, r, q);
 }
 
-enum { TAU_CONST = 0, TAU_PHI = 1, TAU_FIELD = 2 };
+// tau_g: constant; from nu(phi); read from a field (strain Smagorinsky, computed
+// after each step); from nu(phi) and the stored sqrt(Pi:Pi) of the node's
+// non-equilibrium flux (non-equilibrium Smagorinsky, program/LES_2.f90:713-733)
+enum { TAU_CONST = 0, TAU_PHI = 1, TAU_FIELD = 2, TAU_NONEQ = 3 };
 // interface models: the free-energy model carries phi by populations; the
 // conservative Allen-Cahn model (prototype_program/two-phase/dropac.f90) has
 // none and advances phi by a stencil of local equilibria
@@ -149,6 +152,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
         vu2[k] = u2_in[g];
         vu3[k] = u3_in[g];
         if constexpr (TAUMODE == TAU_FIELD) vtau[k] = a.tau_field[g];
+        if constexpr (TAUMODE == TAU_NONEQ) vtau[k] = a.qh_in[g];
     });
 
     // ---------------- phase 1b: issue the 15 upstream population loads of this
@@ -192,7 +196,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             s_u1[h] = vu1[k];
             s_u2[h] = vu2[k];
             s_u3[h] = vu3[k];
-            if constexpr (TAUMODE == TAU_FIELD) s_tau[h] = vtau[k];
+            if constexpr (TAUMODE == TAU_FIELD || TAUMODE == TAU_NONEQ) s_tau[h] = vtau[k];
         }
     });
     __syncthreads();
@@ -243,6 +247,17 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             if (range == 0.0) frac = 0.0;
             const double nu = frac * a.dnu + a.nu1;
             s_tau[h] = 0.5 + 3.0 * nu;
+        }
+        if constexpr (TAUMODE == TAU_NONEQ) {
+            // LES_2.f90:722-730 with sqrt(Q) of the node stored by the last step:
+            // S = (sqrt(nu^2 + 18 Cs^2 D^2 sqrt(Q)) - nu) / (6 Cs^2 D^2), nu_e = Cs^2 D^2 S
+            double frac = (ph - rmin) / range;
+            if (range == 0.0) frac = 0.0;
+            const double nu = frac * a.dnu + a.nu1;
+            double S = sqrt(nu * nu + a.neq_c18 * s_tau[h]) - nu;
+            S = S / a.neq_c6;
+            const double nue = a.neq_cc * S;
+            s_tau[h] = 0.5 + 3.0 * (nu + nue);
         }
     });
     __syncthreads();
@@ -389,6 +404,42 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 }
             }
 
+            if constexpr (TAUMODE == TAU_NONEQ) {
+                // tensor_cal (LES_2.f90:530-551): Pi_ab = sum_i c_ia c_ib (g_i - geq_i) with
+                // this step's g and the equilibria of its start at this node; Q = Pi:Pi
+                // (beta outer, alpha inner); sqrt(Q) is kept for the next step's tau
+                const int hs = (tx + 1) + HX1 * ((ty + 1) + HY1 * (tz + 1));
+                NodeEq<double> nd;
+                nd.p = s_p[hs];
+                nd.u1 = s_u1[hs];
+                nd.u2 = s_u2[hs];
+                nd.u3 = s_u3[hs];
+                const double usq = PRE_U ? s_usq[hs] : nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
+                const double g0 = s_g0[hs], g1 = s_g1[hs], g2 = s_g2[hs];
+                const double s2h = PRE_S ? s_s2h[hs] : 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
+                double t[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+                static_for<0, 15>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    using D = Dir<q>;
+                    constexpr int c[3] = {D::cx, D::cy, D::cz};
+                    const double d = ps[q] - geq<q>(nd, usq, korteweg<q>(g0, g1, g2, s2h), a.kg);
+#pragma unroll
+                    for (int b = 0; b < 3; b++)
+#pragma unroll
+                        for (int al = 0; al < 3; al++) {
+                            const int cc = c[al] * c[b];
+                            if (cc == 1) t[al][b] = t[al][b] + d;
+                            else if (cc == -1) t[al][b] = t[al][b] - d;
+                        }
+                });
+                double Q = 0.0;
+#pragma unroll
+                for (int b = 0; b < 3; b++)
+#pragma unroll
+                    for (int al = 0; al < 3; al++) Q = Q + t[al][b] * t[al][b];
+                a.qh_out[s] = sqrt(Q);
+            }
+
             // moments p = sum g/3, u = sum c g (physics :546-552), order i = 1..15
             double pr = div_const(ps[0], 3.0, 1.0 / 3.0);
             double m1 = 0.0, m2 = 0.0, m3 = 0.0;
@@ -464,14 +515,16 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(
 // ------------------------------------------------------------------ launcher
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUM, int MODEL = MODEL_FE>
 static hipError_t launch_shape(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
-    if constexpr (MODEL != MODEL_FE) {
-        if (batch) return hipErrorInvalidValue;  // ensembles run the free-energy model only
-    }
-    if (batch) {
-        hipLaunchKernelGGL((step_batch_fp64<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(batch->total_tiles),
-                           dim3(2 * TX * TY * TZ), 0, st, (const StepArgs*)batch->args, batch->tile_start,
-                           batch->ninst);
-        return hipGetLastError();
+    if constexpr (MODEL != MODEL_FE || TAUM == TAU_NONEQ) {
+        // ensembles run the free-energy model without LES only
+        if (batch) return hipErrorInvalidValue;
+    } else {
+        if (batch) {
+            hipLaunchKernelGGL((step_batch_fp64<TX, TY, TZ, WALL, FORCE, TAUM>), dim3(batch->total_tiles),
+                               dim3(2 * TX * TY * TZ), 0, st, (const StepArgs*)batch->args, batch->tile_start,
+                               batch->ninst);
+            return hipGetLastError();
+        }
     }
     StepArgs b = a;
     set_tiles(b, TX, TY, TZ);
@@ -483,6 +536,7 @@ static hipError_t launch_shape(const StepArgs& a, hipStream_t st, const BatchLau
 template <int TX, int TY, int TZ, bool WALL, int FORCE>
 static hipError_t launch_tau(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
     if (a.tau_field) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_FIELD>(a, st, batch);
+    if (a.qh_in) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_NONEQ>(a, st, batch);
     if (a.dnu == 0.0) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_CONST>(a, st, batch);
     return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_PHI>(a, st, batch);
 }

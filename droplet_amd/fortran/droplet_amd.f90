@@ -34,8 +34,8 @@ module droplet_amd
         integer(c_int32_t) :: force_kind, force_start
         real(c_double) :: force_amp, force_k, les_cs, les_delta
         integer(c_int32_t) :: les_ranks_x, les_ranks_y, les_ranks_z
-        integer(c_int32_t) :: model
-        integer(c_int32_t) :: reserved(4)
+        integer(c_int32_t) :: model, les_model
+        integer(c_int32_t) :: reserved(3)
         real(c_double) :: ac_a, ac_width
     end type drp_config
 

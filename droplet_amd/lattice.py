@@ -85,6 +85,8 @@ def lib():
         L.drp_scale_interaction.argtypes = [vp, dp, C.c_int]
         L.drp_filter_velocity.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, sz]
         L.drp_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int]
+        L.drp_droplet_size_histogram.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                 C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]
         L.drp_batch_destroy.argtypes = [vp]
         L.drp_batch_destroy.restype = None
         L.drp_tape_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
@@ -253,6 +255,17 @@ class Lattice:
         _check(lib().drp_filter_velocity(self._h, int(kind), int(k), *[_dp(a) for a in out], self.n),
                "drp_filter_velocity")
         return tuple(out)
+
+    def droplet_size_histogram(self, scale: float, bin: float, d_max: float, nbins: int,
+                               threshold: float | None = None):
+        """(counts, total) of diameters 2 r / scale (drp_droplet_size_histogram)."""
+        thr = 0.5 * (self.cfg.phi1 + self.cfg.phi2) if threshold is None else float(threshold)
+        counts = np.zeros(nbins, np.int64)
+        total = C.c_int64(0)
+        _check(lib().drp_droplet_size_histogram(self._h, thr, float(scale), float(bin), float(d_max),
+                                                counts.ctypes.data_as(C.POINTER(C.c_int64)), int(nbins),
+                                                C.byref(total)), "drp_droplet_size_histogram")
+        return counts, total.value
 
     def droplet_census(self, threshold: float | None = None, cap: int = 1 << 16, labels: bool = False):
         """Droplets of the current state (drp_droplet_census): (first, volumes[, labels]),

@@ -69,6 +69,15 @@ extern "C" {
                                    (prototype_program/two-phase/dropac.f90:517-535,640-667,
                                    715-728); FP64, no LES, no body force                */
 
+/* Smagorinsky eddy-viscosity estimates */
+#define DRP_LES_STRAIN 0  /* nu_e = Cs^2 Delta^2 sqrt(2 S:S), S from the D3Q15 velocity
+                             gradient at the end of the step (program/LES_drop.f90:598-693) */
+#define DRP_LES_NONEQ 1   /* nu_e from the non-equilibrium momentum flux of the node,
+                             Pi = sum c c (g - geq), Q = Pi:Pi:
+                             nu_e = Cs^2 Delta^2 (sqrt(nu^2 + 18 Cs^2 Delta^2 sqrt(Q)) - nu)
+                                    / (6 Cs^2 Delta^2)
+                             (program/LES_2.f90:530-551, 713-733); FP64, les_ranks 0 */
+
 /* macroscopic fields for drp_get_macro / drp_set_macro */
 #define DRP_FIELD_PHI 0
 #define DRP_FIELD_P 1
@@ -109,7 +118,8 @@ typedef struct drp_config {
     int32_t les_ranks_y;
     int32_t les_ranks_z;
     int32_t model;        /* DRP_MODEL_*                                               */
-    int32_t reserved[4];  /* must be zero                                              */
+    int32_t les_model;    /* DRP_LES_*: how nu_e is estimated when les_cs, les_delta != 0 */
+    int32_t reserved[3];  /* must be zero                                              */
     double ac_a;          /* Allen-Cahn: relaxation weight A of the phi update; mobility
                              M = (1 - A)/6 (ref dropac.f90 A, M)                       */
     double ac_width;      /* Allen-Cahn: interface width W (ref dropac.f90 W)          */
@@ -365,6 +375,15 @@ int drp_scale_interaction(drp_lattice* lat, double* C, int nshells);
  * interleaved (re, im), x fastest; sign -1 forward, +1 inverse, unnormalised.
  * No reference counterpart (FFTW's role there); for checking the transform. */
 int drp_fft3d(int nx, int ny, int nz, double* data, int sign);
+
+/* Droplet size distribution of the current state: the census above (threshold),
+ * radii r = (3 V / (4 * 3.14))^(1/3) from the cluster volumes as
+ * program/percolation.f90 (:298) forms them, diameters d = 2 r / scale, those
+ * with d > d_max dropped, and the histogram of program/mkdsd.f90 (:68-100):
+ * counts[i] = number of droplets with int(d / bin) == i, i < nbins; *total =
+ * number counted (mkdsd's frequency is counts / total). */
+int drp_droplet_size_histogram(drp_lattice* lat, double threshold, double scale, double bin, double d_max,
+                               int64_t* counts, int64_t nbins, int64_t* total);
 
 /* Message for the calling thread's last failure ("" if none). */
 const char* drp_last_error(void);

@@ -88,6 +88,8 @@ ora_lattice* ora_create(const drp_config* c) {
         if (c->les_ranks_x < 1 || c->les_ranks_y < 1 || c->les_ranks_z < 1) return NULL;
         if (c->nx % c->les_ranks_x || c->ny % c->les_ranks_y || c->nz % c->les_ranks_z) return NULL;
     }
+    if (c->les_model != DRP_LES_STRAIN && c->les_model != DRP_LES_NONEQ) return NULL;
+    if (c->les_model == DRP_LES_NONEQ && (c->les_ranks_x || c->model != DRP_MODEL_FREE_ENERGY)) return NULL;
     if (c->model == DRP_MODEL_ALLEN_CAHN &&
         (c->force_kind != DRP_FORCE_NONE || (c->les_cs != 0.0 && c->les_delta != 0.0) || !(c->ac_width > 0.0)))
         return NULL;
@@ -148,6 +150,7 @@ static void stencils_at(ora_lattice* L, int x, int y, int z) {
 }
 
 static int les_on(const ora_lattice* L) { return L->c.les_cs != 0.0 && L->c.les_delta != 0.0; }
+static int les_noneq(const ora_lattice* L) { return les_on(L) && L->c.les_model == DRP_LES_NONEQ; }
 
 /* Smagorinsky eddy viscosity from gu[k][a] = d_a u_k: program/LES_drop.f90:661-675
    (strain :479-499; loops beta outer, alpha inner) */
@@ -163,7 +166,10 @@ static double eddy_viscosity_of(const ora_lattice* L, double gu[3][3]) {
 }
 
 static double eddy_viscosity(const ora_lattice* L, int64_t s) {
-    if (!les_on(L)) return 0.0;
+    /* the non-equilibrium model has no non-equilibrium part in an initial
+       equilibrium state (its own ini_op, LES_2.f90:322-364, reads geq_procs
+       while that array is still zero; not reproduced) */
+    if (!les_on(L) || les_noneq(L)) return 0.0;
     double gu[3][3];
     for (int k = 0; k < 3; k++)
         for (int a = 0; a < 3; a++) gu[k][a] = L->gu[(k * 3 + a) * L->n + s];
@@ -321,6 +327,36 @@ static void les_tau_update(ora_lattice* L) {
                 double nu = frac * (L->c.nu2 - L->c.nu1) + L->c.nu1;
                 L->tau[s] = 0.5 + 3.0 * (nu + eddy_viscosity_of(L, gu)) + 2.0 / 3.0 * L->c.anu;
             }
+}
+
+/* non-equilibrium Smagorinsky (program/LES_2.f90:713-733): tau for the next step
+   from Pi_ab = sum_i c_ia c_ib (g_i - geq_i) at the node, g after this step's
+   streaming and geq of its start (tensor_cal :530-551, beta outer, alpha inner),
+   Q = Pi:Pi, nu(phi) of the new phi and range */
+static void noneq_tau_update(ora_lattice* L) {
+    const drp_config* c = &L->c;
+    const double Cs = c->les_cs, dl = c->les_delta;
+#pragma omp parallel for num_threads(L->threads) schedule(static)
+    for (int64_t s = 0; s < L->n; s++) {
+        double Q = 0.0;
+        for (int b = 0; b < 3; b++)
+            for (int a = 0; a < 3; a++) {
+                const int* ca = a == 0 ? CX : (a == 1 ? CY : CZ);
+                const int* cb = b == 0 ? CX : (b == 1 ? CY : CZ);
+                double t = 0.0;
+                for (int i = 0; i < 15; i++)
+                    t = t + ((double)ca[i] * (double)cb[i] * (L->g[i * L->n + s] - L->geq[i * L->n + s]));
+                Q = Q + t * t;
+            }
+        double range = L->phi_max - L->phi_min;
+        double frac = (L->phi[s] - L->phi_min) / range;
+        if (range == 0.0) frac = 0.0;
+        double nu = frac * (c->nu2 - c->nu1) + c->nu1;
+        double S = sqrt(nu * nu + 18.0 * Cs * Cs * dl * dl * sqrt(Q)) - nu;
+        S = S / (6.0 * Cs * Cs * dl * dl);
+        double nue = Cs * Cs * dl * dl * S;
+        L->tau[s] = 0.5 + 3.0 * (nu + nue) / 1.0 + 2.0 / 3.0 * c->anu;
+    }
 }
 
 /* moments phi = sum f, p = sum g/3, u = sum c g: physics, program/lbm_drop_xy.f90:541-553 */
@@ -579,7 +615,8 @@ int ora_step(ora_lattice* L, int nsteps) {
         for (int64_t s = 0; s < L->n; s++) moments_at(L, s, allen_cahn(L) ? 2 : 3);
         if (allen_cahn(L)) allen_cahn_phi(L);
         phi_range(L);
-        if (les_on(L)) les_tau_update(L);
+        if (les_noneq(L)) noneq_tau_update(L);
+        else if (les_on(L)) les_tau_update(L);
         L->step = nstep;
     }
     return 0;
@@ -895,6 +932,12 @@ int ora_stencils(ora_lattice* L, double* lap, double* gphi, double* gu) {
 }
 
 /* Equilibria of the current fields and range (equilibrium_cal :421-476) and tau_g. */
+/* the relaxation time tau_g the next step will use */
+int ora_get_tau(const ora_lattice* L, double* tau) {
+    memcpy(tau, L->tau, sizeof(double) * (size_t)L->n);
+    return 0;
+}
+
 int ora_equilibria(ora_lattice* L, double* feq, double* geq, double* tau) {
     /* the relaxation time the next step will use (with LES, from the end of the
        last step) stays untouched */

@@ -65,6 +65,7 @@ def _bind(L):
     L.ora_deformation.argtypes = [vp, C.c_int, dp, dp, dp]
     L.ora_stencils.argtypes = [vp, dp, dp, dp]
     L.ora_equilibria.argtypes = [vp, dp, dp, dp]
+    L.ora_get_tau.argtypes = [vp, dp]
     L.ora_get_force.argtypes = [vp, dp, dp, dp]
     L.ora_spectral_force_fields.argtypes = [vp, dp, dp, dp, dp, dp, dp]
     L.ora_spectral_force_fields.restype = C.c_double
@@ -209,6 +210,12 @@ class OracleLattice:
         gu = np.empty((9,) + self.shape)
         self._L.ora_stencils(self._h, _dp(lap), _dp(gp), _dp(gu))
         return lap, gp, gu.reshape((3, 3) + self.shape)
+
+    def tau(self):
+        """tau_g the next step will use (with LES, the one the last step left)."""
+        out = np.empty(self.shape)
+        self._L.ora_get_tau(self._h, _dp(out))
+        return out
 
     def equilibria(self):
         """(feq [15], geq [15], tau_g) of the current fields."""

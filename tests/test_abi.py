@@ -66,7 +66,7 @@ def test_invalid_config_is_rejected_without_gpu_work():
     (dict(les_ranks_x=8, les_ranks_y=0, les_ranks_z=2), b"all 0 or all positive"),
     (dict(les_ranks_x=3, les_ranks_y=1, les_ranks_z=1), b"do not divide"),
     (dict(les_ranks_x=-1, les_ranks_y=-1, les_ranks_z=-1), b"all 0 or all positive"),
-    (dict(reserved=(C.c_int32 * 4)(0, 0, 0, 7)), b"reserved"),
+    (dict(reserved=(C.c_int32 * 3)(0, 0, 7)), b"reserved"),
     (dict(tau_f=0.5), b"tau_f"),
 ])
 def test_config_fields_are_validated_without_gpu_work(kw, msg):

@@ -120,3 +120,26 @@ def test_census_turbulent_droplets_256():
     print(f"census 256^3 turbulent droplets: {len(f)} droplets, volumes {v.min()}..{v.max()}; "
           f"GPU {t_gpu * 1e3:.2f} ms per call (incl. host sync), restatement {t_cpu * 1e3:.0f} ms (1 core)")
     assert len(f) >= 20
+
+
+def test_droplet_size_histogram():
+    """Droplet size distribution (program/mkdsd.f90:68-100 over the radii
+    program/percolation.f90:298 forms from volumes, r = (3 V / (4 * 3.14))^(1/3)):
+    the GPU census volumes binned exactly as the restatement's volumes binned here,
+    on a layout of spheres of several radii, with a d_max cut."""
+    N = 48
+    rng = np.random.default_rng(21)
+    drops = [(float(x), float(y), float(z), float(r)) for (x, y, z), r in
+             zip(rng.uniform(0, N, (14, 3)), rng.choice([1.5, 2.6, 3.2, 4.1, 5.0, 6.5], 14))]
+    phi = sphere_field(N, drops)
+    lat = _lattice(phi)
+    scale, bin_, dmax, nbins = 1.135, 2.0, 11.0, 12
+    counts, total = lat.droplet_size_histogram(scale, bin_, dmax, nbins, THR)
+    _, vols, _ = census(phi, THR)
+    want = np.zeros(nbins, np.int64)
+    for v in vols:
+        d = 2.0 * (3.0 * v / (4.0 * 3.14)) ** (1.0 / 3.0) / scale
+        if 0 < d <= dmax and int(d / bin_) < nbins:
+            want[int(d / bin_)] += 1
+    np.testing.assert_array_equal(counts, want)
+    assert total == want.sum() and 0 < total < len(vols)   # the cut drops the largest droplets

@@ -236,3 +236,39 @@ def test_batch_fp64_members_vs_oracle(graph):
     for i, (m, o) in enumerate(zip(b.members, oras)):
         o.step(6)
         _assert_same(m, o, f"batch member {i}@26")
+
+
+def test_breakup_time_fp32_vs_fp64():
+    """A shear droplet that breaks up: 96^3, diameter 32, Ca = 1.0 (the first
+    census with more than one droplet, program/percolation.f90's count, comes
+    near step 12000).  The break-up step seen every 50 steps in FP32 shifted
+    storage is within 100 steps of FP64's, and D (mode 2) agrees within 1e-3
+    up to the break-up."""
+    from droplet_amd.lattice import Lattice
+    N, D, Ca = 96, 32.0, 1.0
+    U = _wall_speed(N, D, Ca)
+    drop = [(0.5 * (N - 1),) * 3 + (D,)]
+    l64, l32 = Lattice(_shear(N, U)), Lattice(_shear(N, U, DRP_FP32_SHIFTED))
+    l64.init_droplets(drop)
+    l32.init_droplets(drop)
+    brk, rows, worst = {"fp64": None, "fp32": None}, [], 0.0
+    n = 0
+    while n < 16000 and (brk["fp64"] is None or brk["fp32"] is None):
+        k = 500 if n < 9500 else 50
+        l64.step(k)
+        l32.step(k)
+        n += k
+        d64, d32 = l64.deformation(2)[2], l32.deformation(2)[2]
+        rows.append([n, d64, d32])
+        if brk["fp64"] is None and brk["fp32"] is None:
+            worst = max(worst, abs(d32 - d64))
+        for key, lat in (("fp64", l64), ("fp32", l32)):
+            if brk[key] is None and _breakup(lat):
+                brk[key] = n
+    res = {"N": N, "D": D, "Ca": Ca, "U": U, "breakup_step": brk, "D_abs_err_max_before_breakup": worst,
+           "D_t": rows}
+    _record("breakup_96_Ca1", res)
+    print(json.dumps({k: v for k, v in res.items() if k != "D_t"}))
+    assert brk["fp64"] is not None and brk["fp32"] is not None, brk
+    assert abs(brk["fp32"] - brk["fp64"]) <= 100, brk
+    assert worst <= 1e-3

@@ -349,3 +349,40 @@ def test_planted_mistakes_are_caught(tmp_path, plant, checks):
     lib = pyoracle.build_planted(plant, str(tmp_path))
     for chk in checks:
         assert not chk(lib), f"planted mistake {plant} not caught by {chk.__name__}"
+
+
+def test_noneq_smagorinsky_closed_forms():
+    """Non-equilibrium Smagorinsky (program/LES_2.f90:530-551, 713-733).
+    (a) A uniform state stays at equilibrium, Pi = 0 up to rounding, so tau = 1/2 + 3 nu.
+    (b) Steady Couette flow between walls moving -+U: BGK gives the
+    non-equilibrium flux Pi_xy = Pi_yx = -tau gamma / 3 for the shear rate gamma,
+    so sqrt(Q) = sqrt(2) tau gamma / 3 and tau solves
+    tau = 1/2 + 3 (nu + C S),  S = (sqrt(nu^2 + 18 C sqrt(Q)) - nu) / (6 C),  C = Cs^2 Delta^2.
+    With gamma measured from the profile the interior tau matches that root; the
+    eddy viscosity is half the molecular one here, so a wrong constant (18, 6),
+    a missing square root or the full instead of the non-equilibrium moment
+    misses it by far more than the tolerance."""
+    from droplet_amd.abi import DRP_LES_NONEQ
+    nu, Cs, dl = 0.02, 0.17, 4.0
+    cfg = make_config(6, kappa_g=0.01, nu1=nu, nu2=nu, les_cs=Cs, les_delta=dl, les_model=DRP_LES_NONEQ)
+    o = OracleLattice(cfg, threads=1)
+    o.init_fields(3.0, 1 / 3, 0.02, -0.01, 0.005)
+    o.step(3)
+    assert np.abs(o.tau() - (0.5 + 3 * nu)).max() <= 1e-12   # rounding-level non-equilibrium only
+
+    ny, U = 32, 0.05
+    cfg = make_config(4, ny, 4, kappa_g=0.0, nu1=nu, nu2=nu, wall_axis=DRP_WALL_Y, wall_u=U, les_cs=Cs,
+                      les_delta=dl, les_model=DRP_LES_NONEQ)
+    o = OracleLattice(cfg, threads=1)
+    o.init_fields(np.full(o.shape, cfg.phi1), 1 / 3, 0, 0, 0)
+    o.step(12000)
+    u = o.macro(2)[0, :, 0]
+    gamma = (u[20] - u[11]) / 9.0
+    C = Cs * Cs * dl * dl
+    tau = 0.5 + 3 * nu
+    for _ in range(100):
+        S = (math.sqrt(nu * nu + 18 * C * math.sqrt(2) * tau * gamma / 3) - nu) / (6 * C)
+        tau = 0.5 + 3 * (nu + C * S)
+    got = o.tau()[0, ny // 2, 0]
+    assert got - (0.5 + 3 * nu) > 0.03            # the eddy viscosity matters
+    assert abs(got - tau) <= 1e-6 * tau, (got, tau)
