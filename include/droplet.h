@@ -205,6 +205,17 @@ int drp_set_force(drp_lattice* lat, const double* fx, const double* fy, const do
 /* Diagnostics reductions (wavefront -> workgroup -> grid, FP64 accumulation). */
 int drp_reduce(drp_lattice* lat, drp_stats* stats);
 
+/* drp_reduce without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The partial sums live in one
+ * file-scope device array that each GPU holds until the process ends
+ * (10 MiB + 128 B); lattices of more than DRP_REDUCE_NOALLOC_MAX_NODES nodes get
+ * DRP_ERR_ARG, FP32 lattices DRP_ERR_UNSUPPORTED (their FP64 view allocates).
+ * Returns the same statistics as drp_reduce.  Calls that overlap on one GPU share
+ * the array and race: serialise them. */
+#define DRP_REDUCE_NOALLOC_MAX_NODES ((int64_t)1 << 25)
+int drp_reduce_noalloc(drp_lattice* lat, drp_stats* stats);
+
 /* Taylor deformation D = (L-B)/(L+B) in the shear plane.
  * mode 0: break_up3 (prototype_program/two-phase/dropch.f90:857-948);
  * mode 1: L = 2 max r, B = 4 area/(pi L) (prototype_program/two-phase/drop.f90:439-463);
