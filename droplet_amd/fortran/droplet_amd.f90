@@ -120,6 +120,32 @@ module droplet_amd
             type(c_ptr), value :: lat
             type(drp_stats), intent(out) :: stats
         end function
+        integer(c_int) function drp_reduce_noalloc(lat, stats) bind(C, name="drp_reduce_noalloc")
+            import :: c_int, c_ptr, drp_stats
+            type(c_ptr), value :: lat
+            type(drp_stats), intent(out) :: stats
+        end function
+        integer(c_int) function drp_init_droplets_tanh(lat, drops, ndrops, p_init, width) &
+                bind(C, name="drp_init_droplets_tanh")
+            import :: c_int, c_ptr, c_double, drp_drop
+            type(c_ptr), value :: lat
+            type(drp_drop), intent(in) :: drops(*)
+            integer(c_int), value :: ndrops
+            real(c_double), value :: p_init, width
+        end function
+        ! enesupe_cal (turbulence_hit_xy.f90) and contribution_each_scale (drop_analysis.f90)
+        integer(c_int) function drp_energy_spectrum(lat, E, nshells) bind(C, name="drp_energy_spectrum")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lat
+            real(c_double), intent(out) :: E(*)
+            integer(c_int), value :: nshells
+        end function
+        integer(c_int) function drp_scale_interaction(lat, C, nshells) bind(C, name="drp_scale_interaction")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lat
+            real(c_double), intent(out) :: C(*)
+            integer(c_int), value :: nshells
+        end function
         integer(c_int) function drp_deformation(lat, mode, L, B, D) bind(C, name="drp_deformation")
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: lat

@@ -12,6 +12,12 @@
         Smagorinsky Cs = 0.1, spectral forcing of the shells 1 <= |k| < 3, solenoidal
         k^-5/3 initial velocity with u_rms = 0.03, numpy.random.default_rng(3)):
         MLUPS of the whole step (forcing + LES + step kernels)
+    python -m droplet_amd.probe allen-cahn [--grid 256] [--steps 20]
+        the 256^3 shear droplet of bench.py with the conservative Allen-Cahn phase
+        field (A = 0.4, i.e. mobility 0.1, W = 4, phi in [-0.5, 0.5]): MLUPS
+    python -m droplet_amd.probe spectra [--grid 256] [--reps 3]
+        FFT diagnostics of the 256^3 turbulent-droplet state: energy spectrum,
+        per-scale interaction and a low-pass filter, milliseconds per call
     python -m droplet_amd.probe ensemble [--members 64] [--steps 60] [--fp64]
         64 shear-droplet members, N uniform in {48, 64, 96, 128}, radius N/5,
         Ca uniform in [0.05, 0.5], centre offsets within +-2 cells
@@ -88,15 +94,61 @@ def turbulence(a, dl):
     t_init = time.perf_counter() - t0
     cfg = hit.hit_config(N, We=a.we, D=20.0, precision=DRP_FP32_SHIFTED if a.fp32 else DRP_FP64)
     lat = dl.Lattice(cfg)
+    if a.shape >= 0:
+        lat.tile_shape = a.shape
     lat.init_fields(phi, p, u1, u2, u3)
     del phi, p, u1, u2, u3
     lat.step(3)
     ms = lat.time_steps(a.steps)
     st = lat.reduce()
     print(json.dumps({"workload": f"turbulent droplets {N}^3, 32 x R10, We {a.we}, LES Cs 0.1, spectral forcing kc 3",
-                      "precision": "fp32_shifted" if a.fp32 else "fp64", "steps": a.steps,
+                      "precision": "fp32_shifted" if a.fp32 else "fp64", "steps": a.steps, "shape": lat.tile_shape,
                       "ms_per_step": ms / a.steps, "MLUPS": N ** 3 * a.steps / (ms * 1e-3) / 1e6,
                       "kinetic": st["kinetic"], "phi_mass": st["phi_mass"], "host_init_s": round(t_init, 1)}))
+
+
+def allen_cahn(a, dl):
+    from droplet_amd.abi import DRP_MODEL_ALLEN_CAHN, DRP_WALL_Y, make_config
+    from bench import shear_config
+    base, drops, phys = shear_config(a.grid)
+    cfg = make_config(a.grid, model=DRP_MODEL_ALLEN_CAHN, ac_a=0.4, ac_width=4.0, phi1=-0.5, phi2=0.5,
+                      nu1=0.1, nu2=0.1, kappa_g=base.kappa_g, wall_axis=DRP_WALL_Y, wall_u=base.wall_u)
+    lat = dl.Lattice(cfg)
+    if a.shape >= 0:
+        lat.tile_shape = a.shape
+    lat.init_droplets(drops, 0.0)
+    lat.step(3)
+    ms = lat.time_steps(a.steps)
+    n = a.grid ** 3
+    print(json.dumps({"workload": f"Allen-Cahn shear droplet {a.grid}^3", "shape": lat.tile_shape, "steps": a.steps,
+                      "ms_per_step": ms / a.steps, "MLUPS": n * a.steps / (ms * 1e-3) / 1e6,
+                      "algorithmic_GBs": 320 * n / (ms / a.steps * 1e-3) / 1e9}))
+
+
+def spectra(a, dl):
+    """FFT diagnostics of the turbulent-droplet state, ms per call (host sync included)."""
+    import time
+    from droplet_amd import hit
+    N = a.grid
+    phi, p, u1, u2, u3, _ = hit.initial_state(N, seed=3, u_rms=0.03, count=32, D=20.0)
+    lat = dl.Lattice(hit.hit_config(N, We=a.we, D=20.0))
+    lat.init_fields(phi, p, u1, u2, u3)
+    del phi, p, u1, u2, u3
+    lat.step(2)
+    out = {"grid": N}
+    for name, fn in (("energy_spectrum", lambda: lat.energy_spectrum()),
+                     ("scale_interaction", lambda: lat.scale_interaction()),
+                     ("low_pass_k8", lambda: lat.filter_velocity(0, 8))):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(a.reps):
+            r = fn()
+        out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / a.reps
+    E = lat.energy_spectrum()
+    out["E_shells_1_8"] = [float(v) for v in E[1:9]]
+    out["kinetic"] = lat.reduce()["kinetic"]
+    out["sum_E"] = float(E.sum())
+    print(json.dumps(out))
 
 
 def droplet_census(a, dl):
@@ -123,7 +175,8 @@ def droplet_census(a, dl):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "copysoa", "step", "autotune", "ensemble", "hit", "census"])
+    ap.add_argument("what", choices=["copy", "copysoa", "step", "autotune", "ensemble", "hit", "census", "allen-cahn",
+                                     "spectra"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
@@ -145,6 +198,12 @@ def main(argv=None):
         return
     if a.what == "census":
         droplet_census(a, dl)
+        return
+    if a.what == "allen-cahn":
+        allen_cahn(a, dl)
+        return
+    if a.what == "spectra":
+        spectra(a, dl)
         return
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)

@@ -307,7 +307,10 @@ void drp_tape_destroy(drp_tape* t);
  *   DRP_FILE_FG_G   <dir>/0_<r>_<n>_fg.bin  records (f_i, g_i), g_i used     = step_input input of
  *                   program/LES_drop.f90 (:919-945), which reads outputfg files and keeps g
  * drp_read_reference_files with DRP_FILE_FG is a restart: populations, moments,
- * the step counter (= n) and the phi range of the restored state; with
+ * the step counter (= n) and the phi range of the restored state.  The LES tau
+ * of the restored state is formed with every neighbour current, so a run with
+ * les_ranks set continues bit for bit only where no rank-block halo would have
+ * been one step old (with les_ranks 0 the restart is bit-identical); with
  * DRP_FILE_G or DRP_FILE_FG_G it injects the momentum populations like
  * drp_inject_momentum (read only).
  * DRP_FILE_LAYOUT_XZ, OR-ed into kind, selects program/LES_drop.f90's
