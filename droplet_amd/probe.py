@@ -5,6 +5,10 @@
     python -m droplet_amd.probe copysoa [--grid 256] [--reps 20]
         10, 20 and 40 structure-of-arrays streams of grid^3 doubles (40: floats'
         worth of bytes) copied at once: the achievable rate of the step's pattern
+    python -m droplet_amd.probe yardsticks [--reps 20]
+        independent bandwidth ceilings at 1, 2 and 4 GiB: grid-stride copy,
+        hipMemcpyDtoD, 4x-unrolled non-temporal and plain copies, read only, write
+        only; then the copysoa streams at 256^3
     python -m droplet_amd.probe step [--grid 256] [--steps 20] [--fp32] [--shape S]
         the 256^3 shear-droplet step kernel of bench.py, a few launches
     python -m droplet_amd.probe hit [--grid 256] [--steps 20] [--fp32] [--we 5]
@@ -175,7 +179,7 @@ def droplet_census(a, dl):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["copy", "copysoa", "step", "autotune", "ensemble", "hit", "census", "allen-cahn",
+    ap.add_argument("what", choices=["copy", "copysoa", "yardsticks", "step", "autotune", "ensemble", "hit", "census", "allen-cahn",
                                      "spectra"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
@@ -209,6 +213,14 @@ def main(argv=None):
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
         print(json.dumps({"copy_bytes": a.mib << 20, "reps": a.reps, "GB/s": round(gbs, 1)}))
         return
+    if a.what == "yardsticks":
+        names = ["gridstride", "memcpyDtoD", "unroll4_nt", "unroll4_plain", "read_only", "write_only"]
+        for mib in (1024, 2048, 4096):
+            out = {"MiB": mib}
+            for k, name in enumerate(names):
+                out[name + " GB/s"] = round(dl.copy_bandwidth_kind(mib << 20, k, a.reps), 1)
+            print(json.dumps(out), flush=True)
+        a.what, a.grid = "copysoa", 256
     if a.what == "copysoa":
         n = a.grid ** 3
         out = {"grid": a.grid}

@@ -15,6 +15,7 @@ import torch
 
 from droplet_amd.abi import DRP_FP32_SHIFTED, DRP_WALL_Y, DRP_WALL_Z, make_config, sigma_from_kappa_g
 from oracle import torch_step as ts
+from tests.recording import record
 
 pytestmark = pytest.mark.gpu
 
@@ -227,6 +228,7 @@ def test_adjoint_deformed_drop_central_differences():
     D.close()
     print(f"J = {J.item():.10e}, D = {deformation:.4e}")
     assert deformation > 0.01
+    rows = []
     for k in range(3):
         h = 1e-6 * abs(theta0[k])
         tp, tm = list(theta0), list(theta0)
@@ -235,4 +237,8 @@ def test_adjoint_deformed_drop_central_differences():
         ref = (loss_at(tp) - loss_at(tm)) / (2 * h)
         rel = abs(g[k] - ref) / abs(ref)
         print(f"param {k}: adjoint {g[k]:.10e}  central difference {ref:.10e}  rel {rel:.2e}")
-        assert rel < 1e-5
+        rows.append({"param": ("M", "kappa_g", "wall_u")[k], "adjoint": float(g[k]), "central_difference": ref,
+                     "rel": rel})
+    record("adjoint_deformed_drop", {"steps": nsteps, "K": K, "J": J.item(), "D": deformation, "grad": rows})
+    for r in rows:
+        assert r["rel"] < 1e-5

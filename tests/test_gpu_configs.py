@@ -25,19 +25,12 @@ import pytest
 
 from droplet_amd.abi import DRP_FP32_SHIFTED, DRP_FP64, DRP_WALL_Y, make_config
 from oracle.pyoracle import OracleLattice
+from tests.recording import record
 
 pytestmark = pytest.mark.gpu
 FIELDS = ("phi", "p", "u1", "u2", "u3")
 KAPPA_G = 0.002
 SIGMA = 1.7039 * KAPPA_G ** 0.9991   # sigma = 1.7039 kappa_g^0.9991, program/lbm_drop_xy.f90:51
-
-
-def _record(name, data):
-    d = os.environ.get("DROPLET_RESULTS_DIR")
-    if d:
-        os.makedirs(d, exist_ok=True)
-        with open(os.path.join(d, f"{name}.json"), "w") as fh:
-            json.dump(data, fh, indent=1)
 
 
 def _threads():
@@ -94,7 +87,7 @@ def test_c1_shear_droplet_64():
     table = _run_lockstep(lat, ora, 2000, (1, 100, 2000), "C1")
     st = lat.reduce()
     assert table[-1][2] > 0.0   # sheared drop elongates
-    _record("c1_shear64", {"steps": 2000, "D_mode0_mode2": table, "phi_mass": st["phi_mass"]})
+    record("c1_shear64", {"steps": 2000, "D_mode0_mode2": table, "phi_mass": st["phi_mass"]})
 
 
 def test_c1_tanh_profile():
@@ -116,7 +109,7 @@ def test_c1_tanh_profile():
     ora = OracleLattice(cfg, threads=_threads())
     ora.init_fields(*[f0[k] for k in FIELDS])
     table = _run_lockstep(lat, ora, 300, (1, 300), "C1 tanh")
-    _record("c1_tanh64", {"steps": 300, "D_mode0_mode2": table})
+    record("c1_tanh64", {"steps": 300, "D_mode0_mode2": table})
 
 
 def _breakup(lat):
@@ -154,7 +147,7 @@ def test_c2_fp32_error_256(Ca):
            "u1_linf": float(np.abs(l32.macro(2) - l64.macro(2)).max()),
            "mass_drift_fp64": abs(m64 - m0) / m0, "mass_drift_fp32": abs(m32 - m0) / m0,
            "breakup_step": brk, "D_t": rows}
-    _record(f"c2_fp32_error_256_Ca{Ca}", res)
+    record(f"c2_fp32_error_256_Ca{Ca}", res)
     print(json.dumps({k: v for k, v in res.items() if k != "D_t"}))
     assert worst <= 1e-4, f"Ca {Ca}: FP32 deformation off by {worst:.3e}"
     assert res["mass_drift_fp64"] <= 1e-12
@@ -199,7 +192,7 @@ def test_c4_ensemble_64_members():
         b.close()
     dD = np.abs(np.array(out["fp32"]["D"]) - np.array(out["fp64"]["D"]))
     out["D_abs_err_max"] = float(dD.max())
-    _record("c4_ensemble", out)
+    record("c4_ensemble", out)
     print(json.dumps({k: (v if not isinstance(v, dict) else {kk: vv for kk, vv in v.items() if kk not in ("D", "mass_drift")})
                       for k, v in out.items()}))
     assert max(out["fp64"]["mass_drift"]) <= 1e-12
@@ -267,7 +260,7 @@ def test_breakup_time_fp32_vs_fp64():
                 brk[key] = n
     res = {"N": N, "D": D, "Ca": Ca, "U": U, "breakup_step": brk, "D_abs_err_max_before_breakup": worst,
            "D_t": rows}
-    _record("breakup_96_Ca1", res)
+    record("breakup_96_Ca1", res)
     print(json.dumps({k: v for k, v in res.items() if k != "D_t"}))
     assert brk["fp64"] is not None and brk["fp32"] is not None, brk
     assert abs(brk["fp32"] - brk["fp64"]) <= 100, brk
