@@ -31,6 +31,9 @@ DRP_MODEL_FREE_ENERGY = 0
 DRP_MODEL_ALLEN_CAHN = 1
 DRP_LES_STRAIN = 0
 DRP_LES_NONEQ = 1
+DRP_DERIVED_CHEMICAL_POTENTIAL = 0
+DRP_DERIVED_FREE_ENERGY = 1
+DRP_DERIVED_ADVECTION = 2
 DRP_FILE_FIELDS, DRP_FILE_FG, DRP_FILE_G, DRP_FILE_FG_G = 0, 1, 2, 3
 DRP_FILE_LAYOUT_XZ = 0x100
 
@@ -61,12 +64,13 @@ class Config(C.Structure):
         ("force_amp", C.c_double), ("force_k", C.c_double),
         ("les_cs", C.c_double), ("les_delta", C.c_double),
         ("les_ranks_x", C.c_int32), ("les_ranks_y", C.c_int32), ("les_ranks_z", C.c_int32),
-        ("model", C.c_int32), ("les_model", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("model", C.c_int32), ("les_model", C.c_int32),
+        ("advect_k_high", C.c_int32), ("advect_k_low", C.c_int32), ("advect_start", C.c_int32),
         ("ac_a", C.c_double), ("ac_width", C.c_double),
     ]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Drop(C.Structure):

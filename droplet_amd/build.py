@@ -21,7 +21,7 @@ LIB = os.path.join(OUT_DIR, "libdroplet_amd.so")
 ARCH = os.environ.get("DROPLET_ARCH", "gfx950")
 
 SOURCES = ["step_kernels.hip", "step_fp32.hip", "aux_kernels.hip", "aux_fp32.hip", "adjoint.hip", "spectral.hip", "probe.hip",
-           "census.hip", "noalloc.hip", "spectral_diag.hip",
+           "census.hip", "noalloc.hip", "spectral_diag.hip", "budget.hip",
            "lattice.cpp", "deformation.cpp", "fp32_shifted.cpp", "batch.cpp", "adjoint.cpp", "ref_files.cpp"]
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function", "-Wno-pass-failed",

@@ -74,7 +74,8 @@ extern "C" int drp_tape_create(drp_lattice* L, int K, drp_tape** out) {
     if (L->cfg.precision != DRP_FP64) return drp_fail(DRP_ERR_ARG, "the adjoint needs an FP64 lattice");
     if (L->phys.dnu != 0.0) return drp_fail(DRP_ERR_ARG, "the adjoint needs nu1 == nu2");
     if (L->phys.les) return drp_fail(DRP_ERR_ARG, "the adjoint does not cover the Smagorinsky model");
-    if (L->cfg.model != DRP_MODEL_FREE_ENERGY) return drp_fail(DRP_ERR_ARG, "the adjoint covers the free-energy model only");
+    if (L->cfg.model != DRP_MODEL_FREE_ENERGY || L->cfg.advect_k_high != 0)
+        return drp_fail(DRP_ERR_ARG, "the adjoint covers the free-energy model without scale-separated advection only");
     if (L->cfg.force_kind == DRP_FORCE_SPECTRAL)
         return drp_fail(DRP_ERR_ARG, "the adjoint does not cover the velocity-dependent spectral forcing");
     const int wn = L->cfg.wall_axis == DRP_WALL_Y ? L->cfg.ny : L->cfg.nz;

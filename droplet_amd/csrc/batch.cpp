@@ -195,8 +195,9 @@ extern "C" int drp_batch_create(const drp_config* cfgs, int count, drp_batch** o
     if (cfgs[0].les_cs != 0.0 && cfgs[0].les_delta != 0.0)
         return drp_fail(DRP_ERR_UNSUPPORTED, "LES members are not supported in a batch");
     for (int i = 0; i < count; i++)
-        if (cfgs[i].model != DRP_MODEL_FREE_ENERGY)
-            return drp_fail(DRP_ERR_UNSUPPORTED, "batch members must use the free-energy model");
+        if (cfgs[i].model != DRP_MODEL_FREE_ENERGY || cfgs[i].advect_k_high != 0)
+            return drp_fail(DRP_ERR_UNSUPPORTED, "batch members must use the free-energy model without "
+                                                 "scale-separated advection");
     if (cfgs[0].force_kind == DRP_FORCE_SPECTRAL)
         return drp_fail(DRP_ERR_UNSUPPORTED, "spectral forcing is not supported in a batch");
     drp_batch* B = new (std::nothrow) drp_batch();

@@ -70,6 +70,9 @@ struct StepArgs {
     const double* qh_in;
     double* qh_out;
     double neq_c18, neq_c6, neq_cc;
+    // scale-separated advection: the velocity the phi equilibria carry (u1 | u2 | u3),
+    // or nullptr for u itself
+    const double* uadv;
 };
 
 hipError_t launch_step_fp64(const StepArgs& a, int shape, hipStream_t st, const BatchLaunch* batch = nullptr);
@@ -135,6 +138,10 @@ hipError_t launch_moments_fp64(int nx, int ny, int nz, const double* f, const do
 // blocks) and that step's phi range slot
 hipError_t launch_les_tau_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, const double* mac_old,
                                const LesBlocks& blk, const uint64_t* range, int slot, double* tau, hipStream_t st);
+// derived fields of the analysis programs (budget.hip): kind = DRP_DERIVED_*
+hipError_t launch_derived_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, int kind, double* out,
+                               hipStream_t st);
+
 // diagnostics partial sums: NPART doubles per workgroup
 constexpr int NPART = 10;
 hipError_t launch_reduce_fp64(int nx, int ny, int nz, const Phys& ph, const double* mac, double* partial,
@@ -156,6 +163,10 @@ hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, 
 hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const float* u1,
                                  const float* u2, const float* u3, double* fx, double* fy, double* fz,
                                  hipStream_t st);
+// band-pass of u keeping the shells klo <= int|k| < khi (2 <= khi <= 5; work space of kc = khi)
+hipError_t launch_spectral_band(const SpectralWork& w, int nx, int ny, int nz, int klo, int khi, const double* u1,
+                                const double* u2, const double* u3, double* o1, double* o2, double* o3,
+                                hipStream_t st);
 
 // adjoint.hip: one reverse step of the FP64 step (equal viscosities, no LES)
 struct AdjArgs {

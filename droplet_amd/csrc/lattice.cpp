@@ -122,8 +122,18 @@ static int validate(const drp_config* c) {
         if (c->precision != DRP_FP64 || c->force_kind != DRP_FORCE_NONE || (c->les_cs != 0.0 && c->les_delta != 0.0))
             return drp_fail(DRP_ERR_UNSUPPORTED, "the Allen-Cahn model runs in FP64 without body force or LES");
     }
-    for (size_t k = 0; k < sizeof(c->reserved) / sizeof(c->reserved[0]); k++)
-        if (c->reserved[k] != 0) return drp_fail(DRP_ERR_ARG, "reserved fields must be zero");
+    if (c->advect_k_high != 0) {
+        const int kh = c->advect_k_high, kl = c->advect_k_low;
+        if (kh < 2 || kh > 5) return drp_fail(DRP_ERR_ARG, "advect_k_high must be 0 or 2..5 (got %d)", kh);
+        if (kl < 0 || kl >= kh) return drp_fail(DRP_ERR_ARG, "advect_k_low must lie in 0 .. advect_k_high-1 (got %d)", kl);
+        if (2 * (kh - 1) >= c->nx || 2 * (kh - 1) >= c->ny || 2 * (kh - 1) >= c->nz)
+            return drp_fail(DRP_ERR_ARG, "grid too small for the advecting shells");
+        if (c->precision != DRP_FP64 || c->model != DRP_MODEL_FREE_ENERGY || (c->les_cs != 0.0 && c->les_delta != 0.0))
+            return drp_fail(DRP_ERR_UNSUPPORTED, "scale-separated advection runs in FP64, free-energy model, "
+                                                 "without LES");
+    } else if (c->advect_k_low != 0 || c->advect_start != 0) {
+        return drp_fail(DRP_ERR_ARG, "advect_k_low and advect_start need advect_k_high");
+    }
     {
         const int rx = c->les_ranks_x, ry = c->les_ranks_y, rz = c->les_ranks_z;
         const bool off = rx == 0 && ry == 0 && rz == 0;
@@ -218,24 +228,31 @@ int lattice_alloc(drp_lattice* L) {
         rc = dalloc((void**)&L->force, 3 * (size_t)n * sizeof(double));
         if (!rc) HIP_TRY(hipMemsetAsync(L->force, 0, 3 * (size_t)n * sizeof(double), L->stream));
     }
-    if (!rc && L->cfg.force_kind == DRP_FORCE_SPECTRAL) {
-        const int nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz, kc = (int)L->cfg.force_k;
-        rc = dalloc((void**)&L->spec, spectral_work_doubles(nx, ny, nz, kc) * sizeof(double));
-        if (!rc) {
-            spectral_work_layout(L->specw, L->spec, nx, ny, nz, kc);
-            // e^{2 pi i j / n}, j = 0..n-1, per axis, evaluated on the host
-            const int ns[3] = {nx, ny, nz};
-            double2* dst[3] = {L->specw.twx, L->specw.twy, L->specw.twz};
-            const double pi = std::acos(-1.0);
-            for (int a = 0; a < 3; a++) {
-                std::vector<double2> t(ns[a]);
-                for (int j = 0; j < ns[a]; j++) {
-                    const double th = 2.0 * pi * (double)j / (double)ns[a];
-                    t[j] = make_double2(std::cos(th), std::sin(th));
-                }
-                HIP_TRY(hipMemcpy(dst[a], t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+    // truncated-DFT work space for the modes |k_a| < kc: the spectral forcing
+    // (kc = force_k) and the scale-separated advection (kc = advect_k_high)
+    auto spectral_alloc = [&](double** buf, SpectralWork& w, int kc) -> int {
+        const int nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
+        int r = dalloc((void**)buf, spectral_work_doubles(nx, ny, nz, kc) * sizeof(double));
+        if (r) return r;
+        spectral_work_layout(w, *buf, nx, ny, nz, kc);
+        // e^{2 pi i j / n}, j = 0..n-1, per axis, evaluated on the host
+        const int ns[3] = {nx, ny, nz};
+        double2* dst[3] = {w.twx, w.twy, w.twz};
+        const double pi = std::acos(-1.0);
+        for (int a = 0; a < 3; a++) {
+            std::vector<double2> t(ns[a]);
+            for (int j = 0; j < ns[a]; j++) {
+                const double th = 2.0 * pi * (double)j / (double)ns[a];
+                t[j] = make_double2(std::cos(th), std::sin(th));
             }
+            HIP_TRY(hipMemcpy(dst[a], t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
         }
+        return DRP_OK;
+    };
+    if (!rc && L->cfg.force_kind == DRP_FORCE_SPECTRAL) rc = spectral_alloc(&L->spec, L->specw, (int)L->cfg.force_k);
+    if (!rc && L->cfg.advect_k_high != 0) {
+        rc = spectral_alloc(&L->spec_adv, L->specw_adv, L->cfg.advect_k_high);
+        if (!rc) rc = dalloc((void**)&L->uadv, 3 * (size_t)n * sizeof(double));
     }
     if (!rc && L->cfg.force_kind == DRP_FORCE_TAYLOR_GREEN) {
         const int nx = L->cfg.nx, nz = L->cfg.nz;
@@ -270,10 +287,13 @@ void lattice_free(drp_lattice* L) {
         if (L->qh[k]) (void)hipFree(L->qh[k]);
     if (L->force) (void)hipFree(L->force);
     if (L->spec) (void)hipFree(L->spec);
+    if (L->spec_adv) (void)hipFree(L->spec_adv);
+    if (L->uadv) (void)hipFree(L->uadv);
     if (L->tg) (void)hipFree(L->tg);
     if (L->partial) (void)hipFree(L->partial);
     if (L->mac64) (void)hipFree(L->mac64);
     if (L->census) (void)hipFree(L->census);
+    if (L->budget) (void)hipFree(L->budget);
     if (L->stream && L->owns_stream) (void)hipStreamDestroy(L->stream);
     L->stream = nullptr;
 }
@@ -454,6 +474,7 @@ StepArgs make_step_args(drp_lattice* L, int64_t nstep) {
         a.fy = L->force + n;
         a.fz = L->force + 2 * n;
     }
+    a.uadv = (c.advect_k_high != 0 && nstep >= c.advect_start) ? L->uadv : nullptr;
     a.model = c.model;
     a.ac_a = c.ac_a;
     a.ac_m = (1.0 - c.ac_a) * 1.0 / 6.0;  // M = (1 - A) ds / 6, dropac.f90:65
@@ -473,6 +494,8 @@ extern "C" int drp_step(drp_lattice* L, int nsteps) {
     for (int k = 0; k < nsteps; k++) {
         const int64_t nstep = L->step + 1;
         int rc = spectral_update(L, nstep);
+        if (rc) return rc;
+        rc = advect_update(L, nstep);
         if (rc) return rc;
         StepArgs a = make_step_args(L, nstep);
         const hipError_t e = launch_step_fp64(a, L->shape, L->stream);
@@ -504,6 +527,20 @@ int spectral_update(drp_lattice* L, int64_t nstep) {
         HIP_TRY(launch_spectral_force(L->specw, c.nx, c.ny, c.nz, (int)c.force_k, c.force_amp, m + 2 * n, m + 3 * n,
                                       m + 4 * n, fx, fy, fz, L->stream));
     }
+    return DRP_OK;
+}
+
+// scale-separated advection: from step advect_start on, the band-pass
+// advect_k_low <= int|k| < advect_k_high of the current velocity, which the phi
+// equilibria of the step carry (low_pass at the start of loop iteration n >=
+// step_input, program/lbm_drop_scale_xy.f90:1022-1038)
+int advect_update(drp_lattice* L, int64_t nstep) {
+    const drp_config& c = L->cfg;
+    if (c.advect_k_high == 0 || nstep < c.advect_start) return DRP_OK;
+    const int64_t n = L->n;
+    const double* m = (const double*)L->mac[L->cur];
+    HIP_TRY(launch_spectral_band(L->specw_adv, c.nx, c.ny, c.nz, c.advect_k_low, c.advect_k_high, m + 2 * n,
+                                 m + 3 * n, m + 4 * n, L->uadv, L->uadv + n, L->uadv + 2 * n, L->stream));
     return DRP_OK;
 }
 

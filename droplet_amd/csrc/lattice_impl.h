@@ -28,11 +28,15 @@ struct drp_lattice {
     double* force = nullptr;            // FORCE_FIELD / SPECTRAL: fx | fy | fz
     double* spec = nullptr;             // SPECTRAL: transform work space and twiddle tables
     drp::SpectralWork specw{};
+    double* spec_adv = nullptr;         // scale-separated advection: transform work space
+    drp::SpectralWork specw_adv{};
+    double* uadv = nullptr;             // scale-separated advection: band-passed u1 | u2 | u3
     double* tg = nullptr;               // Taylor-Green sin/cos tables
     double* partial = nullptr;          // diagnostics partial sums
     int64_t partial_cap = 0;
     double* mac64 = nullptr;            // FP32 mode: FP64 copy of the macroscopic fields
     int* census = nullptr;              // droplet census work space (4 n + 4 ints, census.hip)
+    double* budget = nullptr;           // energy-budget work space (19 n doubles + partial sums, budget.hip)
     void* fp32_state = nullptr;         // FP32 mode: shift constants etc.
 };
 
@@ -45,6 +49,8 @@ drp::StepArgs make_step_args(drp_lattice* L, int64_t nstep);
 int default_shape(int precision);
 // SPECTRAL: recompute the force field from the current velocity (FP64 or FP32 fields)
 int spectral_update(drp_lattice* L, int64_t nstep);
+// scale-separated advection: the band-passed velocity the step's phi equilibria carry
+int advect_update(drp_lattice* L, int64_t nstep);
 int reset_range(drp_lattice* L);
 // drp_stats from the compensated sums of the NPART diagnostics and the phi range slot
 void stats_from_sums(const drp_lattice* L, const double* sums, const uint64_t* range_slot, drp_stats* st);

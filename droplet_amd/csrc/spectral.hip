@@ -21,6 +21,10 @@
 // work on N^2 rows or less, one wavefront per output with the lanes over the
 // summed axis.  E_low = 0 gives F = 0 (the reference would divide
 // by zero).
+//
+// The same passes with unit coefficients on the shells klo <= int|k| < khi give
+// the band-pass of u that advects phi in the scale-separated model
+// (program/lbm_drop_scale_xy.f90:846-903, low_pass; u_hat = FFT(u)/N^3 there too).
 #include "kernels.h"
 
 namespace drp {
@@ -128,10 +132,12 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_zsum(int nx, int ny, int nz, cons
     if (lane == 0) U[t] = make_double2(sr * inv, si * inv);
 }
 
-// pass 3b: one workgroup; E_low of the forced shells and the coefficients, in place
+// pass 3b: one workgroup; the coefficients of the kept modes k2lo <= |k|^2 < k2hi,
+// in place, the others zeroed.  Forcing (band = 0): scaled by eps / (2 E_low) with
+// E_low the energy of the kept modes; band-pass (band = 1): unscaled.
 template <int K>
-__global__ __launch_bounds__(SF_BLOCK) void sf_coef(int kc, double eps, double2* __restrict__ C,
-                                                    double* __restrict__ elow_out) {
+__global__ __launch_bounds__(SF_BLOCK) void sf_coef(int k2lo, int k2hi, int band, double eps,
+                                                    double2* __restrict__ C, double* __restrict__ elow_out) {
     constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK, NC = NB * NK;
     __shared__ double2 s_u[NC];
     __shared__ double s_e[SF_BLOCK / 64];
@@ -142,19 +148,19 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_coef(int kc, double eps, double2*
         const int kz = t % NK - K, b = t / NK;
         const int ky = b % NK - K, kx = (b / NK) % (K + 1);
         const int k2 = kx * kx + ky * ky + kz * kz;
-        if (k2 >= 1 && k2 < kc * kc) e += (kx == 0 ? 0.5 : 1.0) * (s.x * s.x + s.y * s.y);
+        if (k2 >= k2lo && k2 < k2hi) e += (kx == 0 ? 0.5 : 1.0) * (s.x * s.x + s.y * s.y);
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
     if ((threadIdx.x & 63) == 0) s_e[threadIdx.x >> 6] = e;
     __syncthreads();
     double elow = 0.0;
     for (int w = 0; w < SF_BLOCK / 64; w++) elow += s_e[w];
-    const double alpha = elow > 0.0 ? 0.5 * eps / elow : 0.0;
+    const double alpha = band ? 1.0 : (elow > 0.0 ? 0.5 * eps / elow : 0.0);
     for (int t = threadIdx.x; t < NC; t += SF_BLOCK) {
         const int kz = t % NK - K, b = t / NK;
         const int ky = b % NK - K, kx = (b / NK) % (K + 1);
         const int k2 = kx * kx + ky * ky + kz * kz;
-        const bool in = k2 >= 1 && k2 < kc * kc;
+        const bool in = k2 >= k2lo && k2 < k2hi;
         C[t] = in ? make_double2(alpha * s_u[t].x, alpha * s_u[t].y) : make_double2(0.0, 0.0);
     }
     if (threadIdx.x == 0) *elow_out = elow;
@@ -224,8 +230,8 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_force(int nx, int ny, int nz, con
 }
 
 template <int K, typename T>
-hipError_t spectral_k(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const T* u1, const T* u2,
-                      const T* u3, double* fx, double* fy, double* fz, hipStream_t st) {
+hipError_t spectral_k(const SpectralWork& w, int nx, int ny, int nz, int k2lo, int k2hi, int band, double eps,
+                      const T* u1, const T* u2, const T* u3, double* fx, double* fy, double* fz, hipStream_t st) {
     constexpr int NK = 2 * K + 1, NB = 3 * (K + 1) * NK;
     const long rows = (long)ny * nz, n = rows * nx;
     hipLaunchKernelGGL((sf_rows_fwd<K, T>), dim3((rows + 3) / 4), dim3(SF_BLOCK), 0, st, nx, ny, nz, u1, u2, u3,
@@ -234,7 +240,7 @@ hipError_t spectral_k(const SpectralWork& w, int nx, int ny, int nz, int kc, dou
     hipLaunchKernelGGL((sf_cols_fwd<K>), dim3((nz * NB + WPB - 1) / WPB), dim3(SF_BLOCK), 0, st, ny, nz, w.A, w.twy,
                        w.B);
     hipLaunchKernelGGL((sf_zsum<K>), dim3((NC + WPB - 1) / WPB), dim3(SF_BLOCK), 0, st, nx, ny, nz, w.B, w.twz, w.C);
-    hipLaunchKernelGGL((sf_coef<K>), dim3(1), dim3(SF_BLOCK), 0, st, kc, eps, w.C, w.elow);
+    hipLaunchKernelGGL((sf_coef<K>), dim3(1), dim3(SF_BLOCK), 0, st, k2lo, k2hi, band, eps, w.C, w.elow);
     hipLaunchKernelGGL((sf_cols_inv<K>), dim3((nz * NB + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st, nz, w.C,
                        w.twz, w.G);
     hipLaunchKernelGGL((sf_rows_inv<K>), dim3((rows * 3 * (K + 1) + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st,
@@ -244,14 +250,16 @@ hipError_t spectral_k(const SpectralWork& w, int nx, int ny, int nz, int kc, dou
     return hipGetLastError();
 }
 
+// modes |k_a| <= kc - 1 transformed; kept: k2lo <= |k|^2 < k2hi
 template <typename T>
-hipError_t spectral_dispatch(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const T* u1,
-                             const T* u2, const T* u3, double* fx, double* fy, double* fz, hipStream_t st) {
+hipError_t spectral_dispatch(const SpectralWork& w, int nx, int ny, int nz, int kc, int k2lo, int k2hi, int band,
+                             double eps, const T* u1, const T* u2, const T* u3, double* fx, double* fy, double* fz,
+                             hipStream_t st) {
     switch (kc - 1) {
-        case 1: return spectral_k<1, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
-        case 2: return spectral_k<2, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
-        case 3: return spectral_k<3, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
-        case 4: return spectral_k<4, T>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+        case 1: return spectral_k<1, T>(w, nx, ny, nz, k2lo, k2hi, band, eps, u1, u2, u3, fx, fy, fz, st);
+        case 2: return spectral_k<2, T>(w, nx, ny, nz, k2lo, k2hi, band, eps, u1, u2, u3, fx, fy, fz, st);
+        case 3: return spectral_k<3, T>(w, nx, ny, nz, k2lo, k2hi, band, eps, u1, u2, u3, fx, fy, fz, st);
+        case 4: return spectral_k<4, T>(w, nx, ny, nz, k2lo, k2hi, band, eps, u1, u2, u3, fx, fy, fz, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -292,13 +300,19 @@ void spectral_work_layout(SpectralWork& w, double* base, int nx, int ny, int nz,
 hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const double* u1,
                                  const double* u2, const double* u3, double* fx, double* fy, double* fz,
                                  hipStream_t st) {
-    return spectral_dispatch<double>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+    return spectral_dispatch<double>(w, nx, ny, nz, kc, 1, kc * kc, 0, eps, u1, u2, u3, fx, fy, fz, st);
 }
 
 hipError_t launch_spectral_force(const SpectralWork& w, int nx, int ny, int nz, int kc, double eps, const float* u1,
                                  const float* u2, const float* u3, double* fx, double* fy, double* fz,
                                  hipStream_t st) {
-    return spectral_dispatch<float>(w, nx, ny, nz, kc, eps, u1, u2, u3, fx, fy, fz, st);
+    return spectral_dispatch<float>(w, nx, ny, nz, kc, 1, kc * kc, 0, eps, u1, u2, u3, fx, fy, fz, st);
+}
+
+hipError_t launch_spectral_band(const SpectralWork& w, int nx, int ny, int nz, int klo, int khi, const double* u1,
+                                const double* u2, const double* u3, double* o1, double* o2, double* o3,
+                                hipStream_t st) {
+    return spectral_dispatch<double>(w, nx, ny, nz, khi, klo * klo, khi * khi, 1, 0.0, u1, u2, u3, o1, o2, o3, st);
 }
 
 }  // namespace drp

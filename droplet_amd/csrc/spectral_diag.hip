@@ -7,6 +7,8 @@
 //   drp_scale_interaction   the per-scale interface-velocity interaction of
 //                           program/drop_analysis.f90:1121-1265:
 //                           C(k) = kappa_g < u^(k) . div(grad phi grad phi) >
+//   drp_phi_contribution    the cross spectrum of u . grad phi and the chemical
+//                           potential, contribution of drop_analysis.f90:476-578
 //
 // The reference transforms with 2decomp/FFTW (r2c with the halved axis z,
 // u_hat = FFT(u) / (nx ny nz), signed wavenumber k = i - n for i >= n/2 + 1).
@@ -194,6 +196,31 @@ __global__ __launch_bounds__(256) void shell_cross(int nx, int ny, int nz, const
     __syncthreads();
     for (int s = threadIdx.x; s < nshells; s += blockDim.x)
         if (bins[s] != 0.0) atomicAdd(&C[s], bins[s]);
+}
+
+// half-space cross spectrum of contribution (drop_analysis.f90:476-578): per shell the
+// sums of w Re(a conj(m)) and of w, w = 1 on the k3 = 0 plane and 2 elsewhere
+__global__ __launch_bounds__(256) void shell_cross_half(int nx, int ny, int nz, const double2* __restrict__ a,
+                                                        const double2* __restrict__ m, int nshells,
+                                                        double* __restrict__ C, double* __restrict__ W) {
+    extern __shared__ double bins[];  // [2][nshells]
+    for (int s = threadIdx.x; s < 2 * nshells; s += blockDim.x) bins[s] = 0.0;
+    __syncthreads();
+    const int64_t half = (int64_t)nx * ny * (nz / 2 + 1);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < half; i += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+        const int s = shell_of(x, y, z, nx, ny, nz);
+        if (s >= nshells) continue;
+        const double w = z == 0 ? 1.0 : 2.0;
+        const double2 av = a[i], mv = m[i];
+        atomicAdd(&bins[s], w * (av.x * mv.x + av.y * mv.y));
+        atomicAdd(&bins[nshells + s], w);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < nshells; s += blockDim.x) {
+        if (bins[s] != 0.0) atomicAdd(&C[s], bins[s]);
+        if (bins[nshells + s] != 0.0) atomicAdd(&W[s], bins[nshells + s]);
+    }
 }
 
 // keep the modes with lo <= int|k| < hi, zero the rest
@@ -457,5 +484,65 @@ extern "C" int drp_scale_interaction(drp_lattice* L, double* C, int nshells) {
     HIP_TRYS(hipMemcpyAsync(h.data(), dC.p, (size_t)nshells * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIP_TRYS(hipStreamSynchronize(L->stream));
     for (int s = 0; s < nshells; s++) C[s] = c.kappa_g * h[s];
+    return DRP_OK;
+}
+
+extern "C" int drp_phi_contribution(drp_lattice* L, double* C, int nshells, double* bands) {
+    if (!L || !C || !bands) return drp_fail(DRP_ERR_ARG, "null argument");
+    if (!L->initialized) return drp_fail(DRP_ERR_STATE, "lattice not initialised");
+    const drp_config& c = L->cfg;
+    const int nb = c.nx / 2 + 1;  // the reference's (xmax+1)/2 + 1 shells
+    if (nshells < 1 || nshells > nb) return drp_fail(DRP_ERR_ARG, "nshells %d not in 1..%d", nshells, nb);
+    FFTPlan P;
+    int rc = make_plan(c.nx, c.ny, c.nz, P);
+    if (rc) return rc;
+    const double* mac = nullptr;
+    rc = velocity_view(L, &mac);
+    if (rc) return rc;
+    const int64_t n = L->n;
+    DevMem ca, tmp, dC;
+    HIP_TRYS(hipMalloc(&ca.p, 2 * (size_t)n * sizeof(double2)));
+    HIP_TRYS(hipMalloc(&tmp.p, (size_t)n * sizeof(double)));
+    HIP_TRYS(hipMalloc(&dC.p, 2 * (size_t)nb * sizeof(double)));
+    double2* adv = (double2*)ca.p;
+    double2* mu = adv + n;
+    const unsigned nblk = (unsigned)((n + 255) / 256);
+    const int kinds[2] = {DRP_DERIVED_ADVECTION, DRP_DERIVED_CHEMICAL_POTENTIAL};
+    double2* dst[2] = {adv, mu};
+    for (int k = 0; k < 2; k++) {
+        HIP_TRYS(launch_derived_fp64(c.nx, c.ny, c.nz, L->phys, mac, kinds[k], (double*)tmp.p, L->stream));
+        hipLaunchKernelGGL(to_complex, dim3(nblk), dim3(256), 0, L->stream, n, (const double*)tmp.p, dst[k]);
+        HIP_TRYS(hipGetLastError());
+        HIP_TRYS(fft3d(P, dst[k], -1, L->stream));
+    }
+    HIP_TRYS(hipMemsetAsync(dC.p, 0, 2 * (size_t)nb * sizeof(double), L->stream));
+    double* dsum = (double*)dC.p;
+    hipLaunchKernelGGL(shell_cross_half, dim3(shell_blocks(n)), dim3(256), 2 * (size_t)nb * sizeof(double), L->stream,
+                       c.nx, c.ny, c.nz, (const double2*)adv, (const double2*)mu, nb, dsum, dsum + nb);
+    HIP_TRYS(hipGetLastError());
+    std::vector<double> h(2 * (size_t)nb);
+    HIP_TRYS(hipMemcpyAsync(h.data(), dsum, h.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIP_TRYS(hipStreamSynchronize(L->stream));
+    std::vector<double> res(nb);
+    for (int s = 0; s < nb; s++) res[s] = h[nb + s] > 0.0 ? h[s] / h[nb + s] : 0.0;
+    for (int s = 0; s < nshells; s++) C[s] = res[s];
+    // energy_large: i = 2..4, energy_medium: i = 5..10, energy_small: i = 11 .. nb (1-based)
+    double all = 0.0, large = 0.0, medium = 0.0, small = 0.0;
+    for (int s = 1; s <= 3 && s < nb; s++) {
+        large = large + res[s];
+        all = all + res[s];
+    }
+    for (int s = 4; s <= 9 && s < nb; s++) {
+        medium = medium + res[s];
+        all = all + res[s];
+    }
+    for (int s = 10; s < nb; s++) {
+        small = small + res[s];
+        all = all + res[s];
+    }
+    bands[0] = all * (-1.0);
+    bands[1] = large * (-1.0);
+    bands[2] = medium * (-1.0);
+    bands[3] = small * (-1.0);
     return DRP_OK;
 }

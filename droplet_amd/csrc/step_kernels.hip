@@ -68,7 +68,9 @@ enum { TAU_CONST = 0, TAU_PHI = 1, TAU_FIELD = 2, TAU_NONEQ = 3 };
 // interface models: the free-energy model carries phi by populations; the
 // conservative Allen-Cahn model (prototype_program/two-phase/dropac.f90) has
 // none and advances phi by a stencil of local equilibria
-enum { MODEL_FE = 0, MODEL_AC = 1 };
+// MODEL_SCALE: the free-energy model whose phi equilibria carry a band-passed
+// velocity (scale-separated advection, StepArgs::uadv)
+enum { MODEL_FE = 0, MODEL_AC = 1, MODEL_SCALE = 2 };
 
 // Resident wavefronts per SIMD the register allocation targets: two 8-wavefront
 // workgroups per CU fit the LDS budget, i.e. 4 per SIMD (<= 128 VGPRs).  The
@@ -174,7 +176,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     // unconditional: threads outside the grid read node (0, 0, 0)'s neighbours
     // (the Allen-Cahn model has no phi populations to load)
     double ps[15];
-    if (MODEL == MODEL_FE || mom) {
+    if (MODEL != MODEL_AC || mom) {
         static_for<0, 15>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             using D = Dir<q>;
@@ -293,6 +295,22 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             vmin = ph;
         } else if (!mom) {
             double ph = 0.0;
+            // MODEL_SCALE: the band-passed velocity of the upstream nodes (equilibrium_cal,
+            // program/lbm_drop_scale_xy.f90:537-540), loaded from L2 UPF directions ahead
+            // of its use into a ring of UPF slots
+            constexpr int UPF = 3;
+            double uring[UPF][3] = {};
+            auto load_uadv = [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
+                double* u = uring[q % UPF];
+                if constexpr (D::cx != 0) u[0] = a.uadv[src];
+                if constexpr (D::cy != 0) u[1] = a.uadv[N + src];
+                if constexpr (D::cz != 0) u[2] = a.uadv[2 * N + src];
+            };
+            if constexpr (MODEL == MODEL_SCALE)
+                static_for<1, 1 + UPF>([&](auto qc) { load_uadv(qc); });
             static_for<0, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
@@ -303,9 +321,18 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 nd.A = s_A[hu];
                 double gt = 0.0;
                 if constexpr (q != 0) {
-                    nd.u1 = s_u1[hu];
-                    nd.u2 = s_u2[hu];
-                    nd.u3 = s_u3[hu];
+                    if constexpr (MODEL == MODEL_SCALE) {
+                        // only the components with c_a != 0 enter c . u
+                        const double* u = uring[q % UPF];
+                        nd.u1 = u[0];
+                        nd.u2 = u[1];
+                        nd.u3 = u[2];
+                        if constexpr (q + UPF < 15) load_uadv(std::integral_constant<int, q + UPF>{});
+                    } else {
+                        nd.u1 = s_u1[hu];
+                        nd.u2 = s_u2[hu];
+                        nd.u3 = s_u3[hu];
+                    }
                     nd.g0 = s_g0[hu];
                     nd.g1 = s_g1[hu];
                     nd.g2 = s_g2[hu];
@@ -548,9 +575,32 @@ constexpr int ac_lds_bytes(int TX, int TY, int TZ, bool var_tau) {
                 (var_tau ? 0 : 1) + 2 * (2 * TX * TY * TZ / 64));
 }
 
+// scale-separated advection: free-energy model without LES, any wall and body force
+template <int TX, int TY, int TZ, bool WALL, int FORCE>
+static hipError_t launch_scale(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
+    if (a.tau_field || a.qh_in) return hipErrorInvalidValue;
+    if (a.dnu == 0.0) return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_CONST, MODEL_SCALE>(a, st, batch);
+    return launch_shape<TX, TY, TZ, WALL, FORCE, TAU_PHI, MODEL_SCALE>(a, st, batch);
+}
+
 template <int TX, int TY, int TZ>
 static hipError_t launch_variant(const StepArgs& a, hipStream_t st, const BatchLaunch* batch) {
     const bool wall = a.wall_axis != 0;
+    if (a.uadv) {
+        switch (a.force_kind) {
+            case FORCE_NONE:
+                return wall ? launch_scale<TX, TY, TZ, true, FORCE_NONE>(a, st, batch)
+                            : launch_scale<TX, TY, TZ, false, FORCE_NONE>(a, st, batch);
+            case FORCE_TAYLOR_GREEN:
+                return wall ? launch_scale<TX, TY, TZ, true, FORCE_TAYLOR_GREEN>(a, st, batch)
+                            : launch_scale<TX, TY, TZ, false, FORCE_TAYLOR_GREEN>(a, st, batch);
+            case FORCE_FIELD:
+                return wall ? launch_scale<TX, TY, TZ, true, FORCE_FIELD>(a, st, batch)
+                            : launch_scale<TX, TY, TZ, false, FORCE_FIELD>(a, st, batch);
+            default:
+                return hipErrorInvalidValue;
+        }
+    }
     if (a.model == MODEL_AC) {  // no body force, no LES; tile shapes whose staging fits in LDS
         if (a.force_kind != FORCE_NONE || a.tau_field) return hipErrorInvalidValue;
         if (a.dnu == 0.0) {

@@ -25,6 +25,8 @@ module droplet_amd
     integer(c_int), parameter :: DRP_FIELD_PHI = 0, DRP_FIELD_P = 1, DRP_FIELD_U1 = 2, DRP_FIELD_U2 = 3, &
                                  DRP_FIELD_U3 = 4
     integer(c_int), parameter :: DRP_POP_PHI = 0, DRP_POP_MOM = 1
+    integer(c_int), parameter :: DRP_DERIVED_CHEMICAL_POTENTIAL = 0, DRP_DERIVED_FREE_ENERGY = 1, &
+                                 DRP_DERIVED_ADVECTION = 2
 
     type, bind(C) :: drp_config
         integer(c_int32_t) :: nx, ny, nz, precision
@@ -35,7 +37,7 @@ module droplet_amd
         real(c_double) :: force_amp, force_k, les_cs, les_delta
         integer(c_int32_t) :: les_ranks_x, les_ranks_y, les_ranks_z
         integer(c_int32_t) :: model, les_model
-        integer(c_int32_t) :: reserved(3)
+        integer(c_int32_t) :: advect_k_high, advect_k_low, advect_start
         real(c_double) :: ac_a, ac_width
     end type drp_config
 
@@ -145,6 +147,25 @@ module droplet_amd
             type(c_ptr), value :: lat
             real(c_double), intent(out) :: C(*)
             integer(c_int), value :: nshells
+        end function
+        integer(c_int) function drp_get_derived(lat, kind, out, n) bind(C, name="drp_get_derived")
+            import :: c_int, c_ptr, c_double, c_size_t
+            type(c_ptr), value :: lat
+            integer(c_int), value :: kind
+            real(c_double), intent(out) :: out(*)
+            integer(c_size_t), value :: n
+        end function
+        integer(c_int) function drp_phi_contribution(lat, C, nshells, bands) bind(C, name="drp_phi_contribution")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lat
+            real(c_double), intent(out) :: C(*)
+            integer(c_int), value :: nshells
+            real(c_double), intent(out) :: bands(4)
+        end function
+        integer(c_int) function drp_step_energy_budget(lat, terms) bind(C, name="drp_step_energy_budget")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: lat
+            real(c_double), intent(out) :: terms(7)
         end function
         integer(c_int) function drp_deformation(lat, mode, L, B, D) bind(C, name="drp_deformation")
             import :: c_int, c_ptr, c_double
@@ -315,7 +336,7 @@ contains
         type(drp_stats) :: s
         integer(c_int) :: v
         v = drp_abi_sizes(sc, sd, ss)
-        drp_layout_ok = (v == 2) .and. (sc == c_sizeof(c)) .and. (sd == c_sizeof(d)) .and. (ss == c_sizeof(s))
+        drp_layout_ok = (v == 3) .and. (sc == c_sizeof(c)) .and. (sd == c_sizeof(d)) .and. (ss == c_sizeof(s))
     end function
 
 end module droplet_amd

@@ -85,6 +85,9 @@ def lib():
         L.drp_scale_interaction.argtypes = [vp, dp, C.c_int]
         L.drp_filter_velocity.argtypes = [vp, C.c_int, C.c_int, dp, dp, dp, sz]
         L.drp_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int]
+        L.drp_get_derived.argtypes = [vp, C.c_int, dp, sz]
+        L.drp_phi_contribution.argtypes = [vp, dp, C.c_int, dp]
+        L.drp_step_energy_budget.argtypes = [vp, dp]
         L.drp_droplet_size_histogram.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                                  C.POINTER(C.c_int64), C.c_int64, C.POINTER(C.c_int64)]
         L.drp_batch_destroy.argtypes = [vp]
@@ -247,6 +250,27 @@ class Lattice:
         ns = self.cfg.nx // 2 + 2 if nshells is None else int(nshells)
         out = np.zeros(ns)
         _check(lib().drp_scale_interaction(self._h, _dp(out), ns), "drp_scale_interaction")
+        return out
+
+    def derived(self, kind: int) -> np.ndarray:
+        """Chemical potential (kind 0), free-energy density (1) or u . grad phi (2) of the
+        current state, [z][y][x] (drp_get_derived)."""
+        out = np.empty(self.shape, np.float64)
+        _check(lib().drp_get_derived(self._h, int(kind), _dp(out), self.n), "drp_get_derived")
+        return out
+
+    def phi_contribution(self, nshells: int | None = None):
+        """(C[s], bands) of drp_phi_contribution; bands = (all, large, medium, small)."""
+        ns = self.cfg.nx // 2 + 1 if nshells is None else int(nshells)
+        out = np.zeros(ns)
+        bands = np.zeros(4)
+        _check(lib().drp_phi_contribution(self._h, _dp(out), ns, _dp(bands)), "drp_phi_contribution")
+        return out, bands
+
+    def step_energy_budget(self) -> np.ndarray:
+        """One step, then its energy-budget terms (left1, left2, right1 .. right5)."""
+        out = np.zeros(7)
+        _check(lib().drp_step_energy_budget(self._h, _dp(out)), "drp_step_energy_budget")
         return out
 
     def filter_velocity(self, kind: int, k: int):

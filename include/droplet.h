@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define DROPLET_ABI_VERSION 2
+#define DROPLET_ABI_VERSION 3
 
 #define DRP_OK 0
 #define DRP_ERR_ARG (-1)         /* invalid argument / size mismatch */
@@ -119,7 +119,15 @@ typedef struct drp_config {
     int32_t les_ranks_z;
     int32_t model;        /* DRP_MODEL_*                                               */
     int32_t les_model;    /* DRP_LES_*: how nu_e is estimated when les_cs, les_delta != 0 */
-    int32_t reserved[3];  /* must be zero                                              */
+    /* Scale-separated advection of phi (program/lbm_drop_scale_xy.f90): from step
+       advect_start on, the phi-population equilibria carry the band-pass of u that
+       keeps the shells advect_k_low <= int|k| < advect_k_high (low_pass :846-903,
+       equilibrium_cal :507-560; the momentum equilibria keep the full u); before it,
+       and with advect_k_high = 0, they carry u itself.  2 <= advect_k_high <= 5,
+       0 <= advect_k_low < advect_k_high; FP64, free-energy model, no LES. */
+    int32_t advect_k_high;
+    int32_t advect_k_low;
+    int32_t advect_start; /* ref step_input                                            */
     double ac_a;          /* Allen-Cahn: relaxation weight A of the phi update; mobility
                              M = (1 - A)/6 (ref dropac.f90 A, M)                       */
     double ac_width;      /* Allen-Cahn: interface width W (ref dropac.f90 W)          */
@@ -398,6 +406,49 @@ int drp_fft3d(int nx, int ny, int nz, double* data, int sign);
  * number counted (mkdsd's frequency is counts / total). */
 int drp_droplet_size_histogram(drp_lattice* lat, double threshold, double scale, double bin, double d_max,
                                int64_t* counts, int64_t nbins, int64_t* total);
+
+/* Derived fields of the droplet analysis programs, for the current state (results
+ * to host, x fastest; FP32 lattices through an FP64 copy of their fields):
+ *   DRP_DERIVED_CHEMICAL_POTENTIAL  mu = T ln(phi/(1 - b phi)) + T/(1 - b phi) - 2 a phi
+ *                                        - kappa_g lap phi   (chemical_potencial_cal,
+ *                                        program/drop_analysis.f90:413-425; lap_cal :383-398)
+ *   DRP_DERIVED_FREE_ENERGY         psi = phi T ln(phi/(1 - b phi)) - a phi^2
+ *                                        (free_energy_cal, :400-411)
+ *   DRP_DERIVED_ADVECTION           u . grad phi (advection_cal, :427-441; grad_cal :362-381) */
+#define DRP_DERIVED_CHEMICAL_POTENTIAL 0
+#define DRP_DERIVED_FREE_ENERGY 1
+#define DRP_DERIVED_ADVECTION 2
+int drp_get_derived(drp_lattice* lat, int kind, double* out, size_t n);
+
+/* Cross spectrum of the phi equation (contribution, program/drop_analysis.f90:476-578):
+ * with A = FFT(u . grad phi) and M = FFT(mu) (unnormalised, the chemical potential
+ * above), C[s] = sum of w Re(A conj(M)) over the half space k3 = 0 .. nz/2 divided by
+ * the sum of w over the same modes, for the shells int|k| == s (signed wavenumbers as
+ * the reference's judge; w = 1 on k3 = 0, else 2), s = 0 .. nshells-1 with nshells <=
+ * nx/2 + 1 (the reference's array; modes of higher shells, which it writes past that
+ * array, are dropped).  bands[4] = the reference's energy_all, energy_large,
+ * energy_medium, energy_small: minus the sums of C over s = 1 .. nx/2, 1..3, 4..9 and
+ * 10 .. nx/2 (bounded by nx/2).  FP64 lattices or FP32 through an FP64 copy. */
+int drp_phi_contribution(drp_lattice* lat, double* C, int nshells, double* bands);
+
+/* One step (as drp_step with nsteps = 1), then the kinetic-energy budget of that step
+ * (program/energy_budget.f90:1251-1700), summed over the nodes, in terms[7]:
+ *   [0] left1  sum |p - (p0 - kappa_g phi lap phi - kappa_g/6 |grad phi|^2)|
+ *   [1] left2  sum u . grad K, K = |u|^2/2
+ *   [2] right1 -sum u . grad p
+ *   [3] right2 sum 2 nu sum_ab d_b(u_a S_ab)
+ *   [4] right3 -sum 2 nu S:S
+ *   [5] right4 -(2/9) kappa_g sum_ab u_a d_b G_ab, G = 4.5 grad phi grad phi - 1.5 |grad phi|^2 I
+ *   [6] right5 sum u . F  (the body force of the step)
+ * with, as the reference evaluates them, p0, lap phi, grad phi, the strain S and G of
+ * the fields the step started from, and p, phi, u and nu(phi) (with the new phi
+ * range) of the fields it produced; D3Q15 stencils over 10 (gradients) and 5
+ * (Laplacian).  Two departures from the reference: K is its commented-out definition
+ * (its active code never assigns K, so its left2 is always 0), and right3 is each
+ * step's own term (the reference never resets its array, so its right3 accumulates
+ * over the run).  FP64, free-energy model; allocates 19 doubles per node on the first
+ * call. */
+int drp_step_energy_budget(drp_lattice* lat, double* terms);
 
 /* Message for the calling thread's last failure ("" if none). */
 const char* drp_last_error(void);

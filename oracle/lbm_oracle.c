@@ -10,6 +10,7 @@
  *   prototype_program/two-phase/dropch.f90 (moving-wall bounce-back, break_up3)
  *   program/LES_drop_xy.f90            (Smagorinsky eddy viscosity)
  *   prototype_program/two-phase/dropac.f90 (conservative Allen-Cahn model)
+ *   program/lbm_drop_scale_xy.f90      (phi advected by a band-pass of u)
  * Every arithmetic expression keeps the reference's evaluation order (Fortran
  * evaluates a chain of + or * left to right), so that on the same inputs the
  * restatement and the compiled reference round identically.
@@ -22,7 +23,9 @@
  * can show each pin catches it: 1 drops the Korteweg term of geq, 2 flips the
  * sign of the body force, 3 divides the Laplacian by 10 instead of 5, 4 moves
  * the bottom wall with +U instead of -U, 5 drops the square root of the
- * interface area's |grad phi|.  Only tests build planted variants.
+ * interface area's |grad phi|, 8 lets the momentum equilibria carry the
+ * band-passed velocity of the scale-separated model too.  Only tests build
+ * planted variants.
  *
  * Parity pins (tests/test_oracle_*.py): golden vectors produced by the compiled
  * reference (tests/golden/), lattice identities, moment identities, polynomial
@@ -67,6 +70,8 @@ typedef struct ora_lattice {
     double *gu;           /* grad u [9][n], gu[k*3+a] = d_a u_k (ref grad_u(k,a)) */
     double *tau;          /* tau_g */
     double *uo1, *uo2, *uo3; /* u at the start of the current step (LES stale halos) */
+    double *ur1, *ur2, *ur3; /* scale-separated advection: band-passed u of the current step */
+    const double* ua[3];     /* the velocity the phi equilibria carry (u, or ur) */
     double phi_min, phi_max; /* range used by nu(phi) in the next step */
     int threads;
 } ora_lattice;
@@ -93,6 +98,9 @@ ora_lattice* ora_create(const drp_config* c) {
     if (c->model == DRP_MODEL_ALLEN_CAHN &&
         (c->force_kind != DRP_FORCE_NONE || (c->les_cs != 0.0 && c->les_delta != 0.0) || !(c->ac_width > 0.0)))
         return NULL;
+    if (c->advect_k_high != 0 && (c->advect_k_low < 0 || c->advect_k_low >= c->advect_k_high ||
+                                  c->model != DRP_MODEL_FREE_ENERGY || (c->les_cs != 0.0 && c->les_delta != 0.0)))
+        return NULL;
     ora_lattice* L = (ora_lattice*)calloc(1, sizeof(ora_lattice));
     if (!L) return NULL;
     L->c = *c;
@@ -101,8 +109,11 @@ ora_lattice* ora_create(const drp_config* c) {
     double** bufs[] = {&L->f, &L->g, &L->fn, &L->gn, &L->feq, &L->geq};
     for (int k = 0; k < 6; k++) *bufs[k] = (double*)calloc((size_t)(15 * n), sizeof(double));
     double** flds[] = {&L->phi, &L->p, &L->u1, &L->u2, &L->u3, &L->fx, &L->fy, &L->fz, &L->lap, &L->tau,
-                       &L->uo1, &L->uo2, &L->uo3};
-    for (int k = 0; k < 13; k++) *flds[k] = (double*)calloc((size_t)n, sizeof(double));
+                       &L->uo1, &L->uo2, &L->uo3, &L->ur1, &L->ur2, &L->ur3};
+    for (int k = 0; k < 16; k++) *flds[k] = (double*)calloc((size_t)n, sizeof(double));
+    L->ua[0] = L->u1;
+    L->ua[1] = L->u2;
+    L->ua[2] = L->u3;
     L->gp = (double*)calloc((size_t)(3 * n), sizeof(double));
     L->gu = (double*)calloc((size_t)(9 * n), sizeof(double));
     L->phi_min = c->phi1;
@@ -114,7 +125,8 @@ ora_lattice* ora_create(const drp_config* c) {
 void ora_destroy(ora_lattice* L) {
     if (!L) return;
     double* all[] = {L->f, L->g, L->fn, L->gn, L->feq, L->geq, L->phi, L->p, L->u1, L->u2, L->u3,
-                     L->fx, L->fy, L->fz, L->lap, L->tau, L->gp, L->gu, L->uo1, L->uo2, L->uo3};
+                     L->fx, L->fy, L->fz, L->lap, L->tau, L->gp, L->gu, L->uo1, L->uo2, L->uo3,
+                     L->ur1, L->ur2, L->ur3};
     for (size_t k = 0; k < sizeof(all) / sizeof(all[0]); k++) free(all[k]);
     free(L);
 }
@@ -288,10 +300,14 @@ static void equilibria_at(ora_lattice* L, int64_t s) {
     for (int i = 0; i < 15; i++) {
         double gt = korteweg_contraction(G, i);
         double ft = (c->anu != 0.0) ? velocity_gradient_contraction(L, s, i) : 0.0;
-        double cu_f = (double)CX[i] * u1 + (double)CY[i] * u2 + (double)CZ[i] * u3;
+        /* phi equilibria: u, or the band-passed u of the scale-separated model
+           (equilibrium_cal, program/lbm_drop_scale_xy.f90:537-540) */
+        double cu_f = (double)CX[i] * L->ua[0][s] + (double)CY[i] * L->ua[1][s] + (double)CZ[i] * L->ua[2][s];
         L->feq[i * L->n + s] = HW[i] * phi + FW[i] * bracket + EW[i] * phi * 3.0 * cu_f
                                + EW[i] * c->kappa_f * gt;                           /* :452-459 */
-        double cu = u1 * (double)CX[i] + u2 * (double)CY[i] + u3 * (double)CZ[i];
+        double cu = ORACLE_PLANT == 8
+                        ? L->ua[0][s] * (double)CX[i] + L->ua[1][s] * (double)CY[i] + L->ua[2][s] * (double)CZ[i]
+                        : u1 * (double)CX[i] + u2 * (double)CY[i] + u3 * (double)CZ[i];
         L->geq[i * L->n + s] = EW[i] * (3.0 * p + 3.0 * cu + 4.5 * (cu * cu) - 1.5 * usq + c->anu * ft)
                                + (ORACLE_PLANT == 1 ? 0.0 : EW[i] * c->kappa_g * gt); /* :461-469 */
     }
@@ -414,20 +430,19 @@ static void taylor_green_force(ora_lattice* L) {
    evaluated per term (FFTW, which the reference links, is not available: parity of this
    routine is pinned by numpy's FFT and the injection invariant <F.u> = eps instead).
    Returns E_low; F = 0 when E_low = 0. */
-double ora_spectral_force_fields(const ora_lattice* L, const double* u1, const double* u2, const double* u3,
-                                 double* fx, double* fy, double* fz) {
+/* DFT coefficients u_hat(k) = sum u e^{-i k.x} / N^3 of the modes |k_a| <= K with
+   k2lo <= |k|^2 < k2hi (direct sums, trigonometric functions per term); re/im
+   [3][NM], NM = (2K+1)^3, zero off the set */
+static void low_mode_coefficients(const ora_lattice* L, const double* const u[3], int K, int k2lo, int k2hi,
+                                  double* re, double* im) {
     const drp_config* c = &L->c;
-    const int kc = (int)c->force_k, K = kc - 1, NK = 2 * K + 1, NM = NK * NK * NK;
+    const int NK = 2 * K + 1, NM = NK * NK * NK;
     const double pi = acos(-1.0), inv = 1.0 / (double)L->n;
-    const double* u[3] = {u1, u2, u3};
-    double* f[3] = {fx, fy, fz};
-    double* re = calloc((size_t)3 * NM, sizeof(double));
-    double* im = calloc((size_t)3 * NM, sizeof(double));
 #pragma omp parallel for num_threads(L->threads) schedule(dynamic)
     for (int m = 0; m < NM; m++) {
         const int kx = m % NK - K, ky = (m / NK) % NK - K, kz = m / (NK * NK) - K;
         const int k2 = kx * kx + ky * ky + kz * kz;
-        if (k2 < 1 || k2 >= kc * kc) continue;
+        if (k2 < k2lo || k2 >= k2hi) continue;
         double sr[3] = {0, 0, 0}, si[3] = {0, 0, 0};
         for (int z = 0; z < c->nz; z++)
             for (int y = 0; y < c->ny; y++)
@@ -445,10 +460,14 @@ double ora_spectral_force_fields(const ora_lattice* L, const double* u1, const d
             im[a * NM + m] = si[a] * inv;
         }
     }
-    double elow = 0.0;
-    for (int a = 0; a < 3; a++)
-        for (int m = 0; m < NM; m++) elow += 0.5 * (re[a * NM + m] * re[a * NM + m] + im[a * NM + m] * im[a * NM + m]);
-    const double alpha = elow > 0.0 ? 0.5 * c->force_amp / elow : 0.0;
+}
+
+/* out = alpha sum over the set of coefficient e^{+i k.x} (the real field of the kept modes) */
+static void low_mode_synthesis(const ora_lattice* L, int K, int k2lo, int k2hi, const double* re, const double* im,
+                               double alpha, double* const out[3]) {
+    const drp_config* c = &L->c;
+    const int NK = 2 * K + 1, NM = NK * NK * NK;
+    const double pi = acos(-1.0);
 #pragma omp parallel for num_threads(L->threads) schedule(static)
     for (int64_t s = 0; s < L->n; s++) {
         const int x = (int)(s % c->nx), y = (int)((s / c->nx) % c->ny), z = (int)(s / ((int64_t)c->nx * c->ny));
@@ -456,16 +475,51 @@ double ora_spectral_force_fields(const ora_lattice* L, const double* u1, const d
         for (int m = 0; m < NM; m++) {
             const int kx = m % NK - K, ky = (m / NK) % NK - K, kz = m / (NK * NK) - K;
             const int k2 = kx * kx + ky * ky + kz * kz;
-            if (k2 < 1 || k2 >= kc * kc) continue;
+            if (k2 < k2lo || k2 >= k2hi) continue;
             const double th = 2.0 * pi * ((double)kx * x / c->nx + (double)ky * y / c->ny + (double)kz * z / c->nz);
             const double cs = cos(th), sn = sin(th);
             for (int a = 0; a < 3; a++) v[a] += re[a * NM + m] * cs - im[a * NM + m] * sn;
         }
-        for (int a = 0; a < 3; a++) f[a][s] = alpha * v[a];
+        for (int a = 0; a < 3; a++) out[a][s] = alpha * v[a];
     }
+}
+
+double ora_spectral_force_fields(const ora_lattice* L, const double* u1, const double* u2, const double* u3,
+                                 double* fx, double* fy, double* fz) {
+    const drp_config* c = &L->c;
+    const int kc = (int)c->force_k, K = kc - 1, NK = 2 * K + 1, NM = NK * NK * NK;
+    const double* u[3] = {u1, u2, u3};
+    double* const f[3] = {fx, fy, fz};
+    double* re = calloc((size_t)3 * NM, sizeof(double));
+    double* im = calloc((size_t)3 * NM, sizeof(double));
+    low_mode_coefficients(L, u, K, 1, kc * kc, re, im);
+    double elow = 0.0;
+    for (int a = 0; a < 3; a++)
+        for (int m = 0; m < NM; m++) elow += 0.5 * (re[a * NM + m] * re[a * NM + m] + im[a * NM + m] * im[a * NM + m]);
+    const double alpha = elow > 0.0 ? 0.5 * c->force_amp / elow : 0.0;
+    low_mode_synthesis(L, K, 1, kc * kc, re, im, alpha, f);
     free(re);
     free(im);
     return elow;
+}
+
+/* Band-pass of u keeping klo <= int|k| < khi, low_pass of program/lbm_drop_scale_xy.f90
+   (:846-903: u_hat = FFT(u)/N^3 zeroed where int|k| + 1 < klo + 1 or >= khi + 1, then
+   transformed back); the same direct DFT as the forcing.  Returns 0, or -1 for
+   khi outside 1..(grid)/2. */
+int ora_band_pass(const ora_lattice* L, const double* u1, const double* u2, const double* u3, int klo, int khi,
+                  double* o1, double* o2, double* o3) {
+    if (khi < 1 || klo < 0 || klo >= khi) return -1;
+    const int K = khi - 1, NK = 2 * K + 1, NM = NK * NK * NK;
+    const double* u[3] = {u1, u2, u3};
+    double* const o[3] = {o1, o2, o3};
+    double* re = calloc((size_t)3 * NM, sizeof(double));
+    double* im = calloc((size_t)3 * NM, sizeof(double));
+    low_mode_coefficients(L, u, K, klo * klo, khi * khi, re, im);
+    low_mode_synthesis(L, K, klo * klo, khi * khi, re, im, 1.0, o);
+    free(re);
+    free(im);
+    return 0;
 }
 
 /* ---------------------------------------------------------------- initialisation */
@@ -586,6 +640,23 @@ static void allen_cahn_phi(ora_lattice* L) {
     memcpy(L->phi, next, sizeof(double) * (size_t)n);
 }
 
+/* the velocity the phi equilibria of step nstep carry: the band-pass of u from
+   advect_start on (low_pass at loop iteration n >= step_input,
+   program/lbm_drop_scale_xy.f90:1022-1038), u itself otherwise */
+static void advect_velocity(ora_lattice* L, int64_t nstep) {
+    const drp_config* c = &L->c;
+    if (c->advect_k_high != 0 && nstep >= c->advect_start) {
+        ora_band_pass(L, L->u1, L->u2, L->u3, c->advect_k_low, c->advect_k_high, L->ur1, L->ur2, L->ur3);
+        L->ua[0] = L->ur1;
+        L->ua[1] = L->ur2;
+        L->ua[2] = L->ur3;
+    } else {
+        L->ua[0] = L->u1;
+        L->ua[1] = L->u2;
+        L->ua[2] = L->u3;
+    }
+}
+
 int ora_step(ora_lattice* L, int nsteps) {
     const drp_config* c = &L->c;
     for (int k = 0; k < nsteps; k++) {
@@ -600,6 +671,7 @@ int ora_step(ora_lattice* L, int nsteps) {
             ora_spectral_force_fields(L, L->u1, L->u2, L->u3, L->fx, L->fy, L->fz);
             force_on = 1;
         }
+        advect_velocity(L, nstep);
         /* various_cal, gphi_cal, equilibrium_cal (+ tau of physics; with LES tau comes
            from the end of the previous step) */
         compute_equilibria(L, !les_on(L));
@@ -943,6 +1015,7 @@ int ora_equilibria(ora_lattice* L, double* feq, double* geq, double* tau) {
        last step) stays untouched */
     double* kept = (double*)malloc(sizeof(double) * (size_t)L->n);
     memcpy(kept, L->tau, sizeof(double) * (size_t)L->n);
+    advect_velocity(L, L->step + 1);   /* the equilibria of the next step */
     compute_equilibria(L, 1);
     memcpy(feq, L->feq, sizeof(double) * 15 * (size_t)L->n);
     memcpy(geq, L->geq, sizeof(double) * 15 * (size_t)L->n);

@@ -28,7 +28,7 @@ def build_planted(plant: int, out_dir: str) -> str:
     out = os.path.join(out_dir, f"liblbm_oracle_plant{plant}.so")
     subprocess.run(["gcc", "-O2", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c11", f"-DORACLE_PLANT={plant}",
                     "-shared", "-o", out, os.path.join(HERE, "lbm_oracle.c"), os.path.join(HERE, "census_oracle.c"),
-                    "-lm"], check=True)
+                    os.path.join(HERE, "budget_oracle.c"), "-lm"], check=True)
     return out
 
 
@@ -38,7 +38,9 @@ def lib(path: str | None = None):
     if path is not None:
         return _bind(C.CDLL(path))
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        srcs = [os.path.join(HERE, f) for f in ("lbm_oracle.c", "census_oracle.c", "budget_oracle.c")]
+        srcs.append(os.path.join(HERE, "..", "include", "droplet.h"))
+        if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < max(os.path.getmtime(f) for f in srcs):
             build()
         _lib = _bind(C.CDLL(LIB_PATH))
     return _lib
@@ -72,6 +74,9 @@ def _bind(L):
     i64p, i32p = C.POINTER(C.c_int64), C.POINTER(C.c_int32)
     L.ora_census.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_double, i64p, i64p, i64p, C.c_int64, i32p]
     L.ora_percolation_count.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_double, i64p]
+    L.ora_band_pass.argtypes = [vp, dp, dp, dp, C.c_int, C.c_int, dp, dp, dp]
+    L.ora_derived.argtypes = [C.POINTER(Config), dp, C.c_int, dp]
+    L.ora_budget_terms.argtypes = [C.POINTER(Config), dp, dp, dp, C.c_double, C.c_double, dp]
     return L
 
 
@@ -103,6 +108,40 @@ def percolation_count(phi: np.ndarray, threshold: float) -> int:
     if rc:
         raise RuntimeError("percolation scan needs more than 1000 provisional labels" if rc == -2 else "ora_percolation_count")
     return out.value
+
+
+def _mac(cfg: Config, fields) -> np.ndarray:
+    """phi, p, u1, u2, u3 (each broadcast to [z][y][x]) as one contiguous [5][n] array."""
+    shape = (cfg.nz, cfg.ny, cfg.nx)
+    return np.ascontiguousarray(np.stack([np.broadcast_to(np.asarray(f, np.float64), shape) for f in fields]))
+
+
+def derived(cfg: Config, fields, kind: int, libpath: str | None = None) -> np.ndarray:
+    """Chemical potential (0), free-energy density (1) or u . grad phi (2) of the fields
+    (phi, p, u1, u2, u3), restated (budget_oracle.c)."""
+    L = lib(libpath) if libpath else lib()
+    m = _mac(cfg, fields)
+    out = np.empty(m.shape[1:])
+    if L.ora_derived(C.byref(cfg), _dp(m), int(kind), _dp(out)):
+        raise ValueError("ora_derived")
+    return out
+
+
+def budget_terms(cfg: Config, old, new, force=None, phi_range=None, libpath: str | None = None) -> np.ndarray:
+    """The seven energy-budget terms of one step from the fields it started from (old)
+    and produced (new), each (phi, p, u1, u2, u3); force (fx, fy, fz) or None;
+    phi_range (min, max) of the new phi, default the clamped extremes (budget_oracle.c)."""
+    L = lib(libpath) if libpath else lib()
+    mo, mn = _mac(cfg, old), _mac(cfg, new)
+    f = None if force is None else _mac(cfg, tuple(force) + (0.0, 0.0))[:3].copy()
+    if phi_range is None:
+        phi_range = (min(cfg.phi2, float(mn[0].min())), max(cfg.phi1, float(mn[0].max())))
+    out = np.zeros(7)
+    rc = L.ora_budget_terms(C.byref(cfg), _dp(mo), _dp(mn), None if f is None else _dp(f), float(phi_range[0]),
+                            float(phi_range[1]), _dp(out))
+    if rc:
+        raise ValueError("ora_budget_terms")
+    return out
 
 
 def _dp(a: np.ndarray):
@@ -185,6 +224,15 @@ class OracleLattice:
         out = [np.empty(self.shape, np.float64) for _ in range(3)]
         elow = self._L.ora_spectral_force_fields(self._h, *[_dp(a) for a in us], *[_dp(a) for a in out])
         return out[0], out[1], out[2], elow
+
+    def band_pass(self, u1, u2, u3, klo: int, khi: int):
+        """The restated low_pass (program/lbm_drop_scale_xy.f90:846-903): the shells
+        klo <= int|k| < khi of (u1, u2, u3)."""
+        us = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), self.shape)) for a in (u1, u2, u3)]
+        out = [np.empty(self.shape, np.float64) for _ in range(3)]
+        if self._L.ora_band_pass(self._h, *[_dp(a) for a in us], int(klo), int(khi), *[_dp(a) for a in out]):
+            raise ValueError("ora_band_pass")
+        return out
 
     def phi_range(self):
         mn, mx = C.c_double(), C.c_double()

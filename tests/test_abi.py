@@ -34,7 +34,7 @@ def test_struct_layouts_match():
     lib = C.CDLL(LIB)
     sz = [C.c_size_t() for _ in range(3)]
     ver = lib.drp_abi_sizes(*[C.byref(s) for s in sz])
-    assert ver == 2
+    assert ver == 3
     assert sz[0].value == C.sizeof(abi.Config)
     assert sz[1].value == C.sizeof(abi.Drop)
     assert sz[2].value == C.sizeof(abi.Stats)
@@ -66,7 +66,8 @@ def test_invalid_config_is_rejected_without_gpu_work():
     (dict(les_ranks_x=8, les_ranks_y=0, les_ranks_z=2), b"all 0 or all positive"),
     (dict(les_ranks_x=3, les_ranks_y=1, les_ranks_z=1), b"do not divide"),
     (dict(les_ranks_x=-1, les_ranks_y=-1, les_ranks_z=-1), b"all 0 or all positive"),
-    (dict(reserved=(C.c_int32 * 3)(0, 0, 7)), b"reserved"),
+    (dict(advect_k_high=6), b"advect_k_high"),
+    (dict(advect_k_high=2, advect_k_low=2), b"advect_k_low"),
     (dict(tau_f=0.5), b"tau_f"),
 ])
 def test_config_fields_are_validated_without_gpu_work(kw, msg):
