@@ -134,20 +134,21 @@ __global__ void les_tau_fp32(int nx, int ny, int nz, Phys P, const float* __rest
     tau[s] = (float)(0.5 + 3.0 * (nu + P.les_c * sqrt(2.0 * (double)sum)));
 }
 
-// The same tau with every neighbour current, marching in z over a 64 x 4 column
+// The same tau with every neighbour current, marching in z over a 64 x 8 column
 // of nodes with the planes z-1, z, z+1 of u (one-node x-y halo) in a ring of
-// three LDS planes; the arithmetic and its order are those of les_tau_fp32.
-constexpr int ZM32_TX = 64, ZM32_TY = 4, ZM32_ZC = 16;
+// three LDS planes; the loads of plane z+2 are in flight into registers while
+// plane z is computed.  The arithmetic and its order are those of les_tau_fp32.
+constexpr int ZM32_TX = 64, ZM32_TY = 8, ZM32_ZC = 16;
+constexpr int ZM32_HX = ZM32_TX + 2, ZM32_HP = ZM32_HX * (ZM32_TY + 2), ZM32_NT = ZM32_TX * ZM32_TY;
+constexpr int ZM32_PER = (ZM32_HP + ZM32_NT - 1) / ZM32_NT;
 __device__ __forceinline__ int wrap_zm32(int v, int n, bool small) {
     if (small) return ((v % n) + n) % n;
     return v < 0 ? v + n : (v >= n ? v - n : v);
 }
-__global__ __launch_bounds__(ZM32_TX* ZM32_TY) void les_tau_fp32_zm(int nx, int ny, int nz, Phys P,
-                                                                     const float* __restrict__ mac,
-                                                                     const uint64_t* __restrict__ range, int slot,
-                                                                     float* __restrict__ tau, int tiles_x, int tiles_y) {
-    constexpr int HX = ZM32_TX + 2, HP = HX * (ZM32_TY + 2);
-    __shared__ float su[3][3][HP];  // [ring slot][component][plane node]
+__global__ __launch_bounds__(ZM32_NT) void les_tau_fp32_zm(int nx, int ny, int nz, Phys P, const float* __restrict__ mac,
+                                                          const uint64_t* __restrict__ range, int slot,
+                                                          float* __restrict__ tau, int tiles_x, int tiles_y) {
+    __shared__ float su[3][3][ZM32_HP];  // [ring slot][component][plane node]
     const int64_t N = (int64_t)nx * ny * nz;
     const int x0 = (blockIdx.x % tiles_x) * ZM32_TX, y0 = ((blockIdx.x / tiles_x) % tiles_y) * ZM32_TY;
     const int zs = (blockIdx.x / (tiles_x * tiles_y)) * ZM32_ZC, ze = min(zs + ZM32_ZC, nz);
@@ -157,17 +158,39 @@ __global__ __launch_bounds__(ZM32_TX* ZM32_TY) void les_tau_fp32_zm(int nx, int 
     const bool in = x < nx && y < ny;
     const double rmax = from_ordered_bits(range[2 * slot]), rmin = from_ordered_bits(range[2 * slot + 1]);
     const double rg = rmax - rmin;
-    auto load = [&](int zg, float (*dst)[HP]) {
+    auto fetch = [&](int zg, float (&r)[ZM32_PER][3]) {
         const int64_t zoff = (int64_t)nx * ny * wrap_zm32(zg, nz, small);
-        for (int h = threadIdx.x; h < HP; h += ZM32_TX * ZM32_TY) {
-            const int64_t g = wrap_zm32(x0 + h % HX - 1, nx, small) + (int64_t)nx * wrap_zm32(y0 + h / HX - 1, ny, small) + zoff;
-            for (int k = 0; k < 3; k++) dst[k][h] = mac[(2 + k) * N + g];
+#pragma unroll
+        for (int i = 0; i < ZM32_PER; i++) {
+            const int h0 = threadIdx.x + i * ZM32_NT, h = h0 < ZM32_HP ? h0 : ZM32_HP - 1;
+            const int64_t g = wrap_zm32(x0 + h % ZM32_HX - 1, nx, small) +
+                              (int64_t)nx * wrap_zm32(y0 + h / ZM32_HX - 1, ny, small) + zoff;
+#pragma unroll
+            for (int k = 0; k < 3; k++) r[i][k] = mac[(2 + k) * N + g];
         }
     };
-    load(zs - 1, su[0]);
-    load(zs, su[1]);
+    auto store = [&](const float (&r)[ZM32_PER][3], float (*dst)[ZM32_HP]) {
+#pragma unroll
+        for (int i = 0; i < ZM32_PER; i++) {
+            const int h = threadIdx.x + i * ZM32_NT;
+            if (h < ZM32_HP)
+#pragma unroll
+                for (int k = 0; k < 3; k++) dst[k][h] = r[i][k];
+        }
+    };
+    float next[ZM32_PER][3];
+    // ring slot of plane z' is (z' - zs + 1) mod 3
+    for (int p = 0; p < 3; p++) {
+        fetch(zs - 1 + p, next);
+        store(next, su[p]);
+    }
+    const int64_t col = (int64_t)(in ? x : 0) + (int64_t)nx * (in ? y : 0);
+    float phi_next = mac[col + (int64_t)nx * ny * zs];
     for (int z = zs; z < ze; z++) {
-        load(z + 1, su[(z - zs + 2) % 3]);
+        const bool more = z + 1 < ze;  // plane z+2 is needed by plane z+1
+        if (more) fetch(z + 2, next);
+        const float phit = phi_next;
+        if (more) phi_next = mac[col + (int64_t)nx * ny * (z + 1)];
         __syncthreads();
         if (in) {
             const int rz[3] = {(z - zs) % 3, (z - zs + 1) % 3, (z - zs + 2) % 3};  // planes z-1, z, z+1
@@ -175,7 +198,7 @@ __global__ __launch_bounds__(ZM32_TX* ZM32_TY) void les_tau_fp32_zm(int nx, int 
             sforx<1, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
-                const int h = (tx + 1 + D::cx) + HX * (ty + 1 + D::cy);
+                const int h = (tx + 1 + D::cx) + ZM32_HX * (ty + 1 + D::cy);
                 for (int k = 0; k < 3; k++) {
                     const float v = su[rz[1 + D::cz]][k][h];
                     gu[k][0] = acc<D::cx>(gu[k][0], v);
@@ -189,13 +212,13 @@ __global__ __launch_bounds__(ZM32_TX* ZM32_TY) void les_tau_fp32_zm(int nx, int 
                     const float sab = 0.05f * (gu[a][b] + gu[b][a]);
                     sum += sab * sab;
                 }
-            const int64_t s = x + (int64_t)nx * (y + (int64_t)ny * z);
-            const double phi = P.phi1 + (double)mac[s];
+            const double phi = P.phi1 + (double)phit;
             const double frac = rg == 0.0 ? 0.0 : (phi - rmin) / rg;
             const double nu = frac * P.dnu + P.nu1;
-            tau[s] = (float)(0.5 + 3.0 * (nu + P.les_c * sqrt(2.0 * (double)sum)));
+            tau[col + (int64_t)nx * ny * z] = (float)(0.5 + 3.0 * (nu + P.les_c * sqrt(2.0 * (double)sum)));
         }
         __syncthreads();
+        if (more) store(next, su[(z - zs) % 3]);  // the slot of plane z-1
     }
 }
 
@@ -230,8 +253,8 @@ hipError_t launch_les_tau_fp32(int nx, int ny, int nz, const Phys& ph, const flo
                            tau);
     } else {
         const int mx = (nx + ZM32_TX - 1) / ZM32_TX, my = (ny + ZM32_TY - 1) / ZM32_TY, mz = (nz + ZM32_ZC - 1) / ZM32_ZC;
-        hipLaunchKernelGGL(les_tau_fp32_zm, dim3((unsigned)(mx * my * mz)), dim3(ZM32_TX * ZM32_TY), 0, st, nx, ny, nz,
-                           ph, mac, range, slot, tau, mx, my);
+        hipLaunchKernelGGL(les_tau_fp32_zm, dim3((unsigned)(mx * my * mz)), dim3(ZM32_NT), 0, st, nx, ny, nz, ph, mac,
+                           range, slot, tau, mx, my);
     }
     return hipGetLastError();
 }

@@ -208,27 +208,50 @@ __global__ __launch_bounds__(LES_TX* LES_TY* LES_TZ) void les_tau_fp64(Grid G, P
 // The same tau with every neighbour current (no rank blocks), marching in z: a
 // workgroup owns a ZM_TX x ZM_TY column of nodes over ZM_ZC planes and keeps the
 // planes z-1, z, z+1 of u (with a one-node x-y halo) in a ring of three LDS
-// planes, loading one new plane per step in z.  u is read from L2 / HBM about
-// (66 x 6)/(64 x 4) (ZM_ZC + 2)/ZM_ZC times instead of the 3.2 times of
-// 32 x 4 x 2 tiles.  Sums are formed in the order of les_tau_fp64, so both give
-// identical tau.
-constexpr int ZM_TX = 64, ZM_TY = 4, ZM_ZC = 16;
+// planes.  While plane z is computed, the loads of plane z+2 are already in
+// flight into registers (stored into the ring slot of plane z-1 afterwards), so
+// their latency overlaps the stencil work.  u is read from L2 / HBM about
+// (66 x 10)/(64 x 8) (ZM_ZC + 2)/ZM_ZC times.  Sums are formed in the order of
+// les_tau_fp64 (the divisions by 10 correctly rounded), so both give identical tau.
+constexpr int ZM_TX = 64, ZM_TY = 8, ZM_ZC = 16;
+constexpr int ZM_HX = ZM_TX + 2, ZM_HP = ZM_HX * (ZM_TY + 2), ZM_NT = ZM_TX * ZM_TY;
+constexpr int ZM_PER = (ZM_HP + ZM_NT - 1) / ZM_NT;  // plane nodes per thread
+
+template <typename T> struct ZmPlane {
+    T v[ZM_PER][3];
+};
+
 template <typename T>
-__device__ __forceinline__ void zm_load_plane(const Grid& G, const T* __restrict__ mac, int64_t N, int x0, int y0,
-                                              int zg, bool small, T (*dst)[(ZM_TX + 2) * (ZM_TY + 2)]) {
-    constexpr int HX = ZM_TX + 2, HP = HX * (ZM_TY + 2);
+__device__ __forceinline__ ZmPlane<T> zm_fetch(const Grid& G, const T* __restrict__ mac, int64_t N, int x0, int y0,
+                                               int zg, bool small) {
+    ZmPlane<T> r;
     const int64_t zoff = (int64_t)G.nx * G.ny * wrap_any(zg, G.nz, small);
-    for (int h = threadIdx.x; h < HP; h += ZM_TX * ZM_TY) {
-        const int64_t g = wrap_any(x0 + h % HX - 1, G.nx, small) + (int64_t)G.nx * wrap_any(y0 + h / HX - 1, G.ny, small) + zoff;
-        for (int k = 0; k < 3; k++) dst[k][h] = mac[(2 + k) * N + g];
+#pragma unroll
+    for (int i = 0; i < ZM_PER; i++) {
+        // unconditional loads (index clamped into the plane) keep the wait a straight count
+        const int h0 = threadIdx.x + i * ZM_NT, h = h0 < ZM_HP ? h0 : ZM_HP - 1;
+        const int64_t g = wrap_any(x0 + h % ZM_HX - 1, G.nx, small) +
+                          (int64_t)G.nx * wrap_any(y0 + h / ZM_HX - 1, G.ny, small) + zoff;
+#pragma unroll
+        for (int k = 0; k < 3; k++) r.v[i][k] = mac[(2 + k) * N + g];
+    }
+    return r;
+}
+
+template <typename T> __device__ __forceinline__ void zm_store(const ZmPlane<T>& r, T (*dst)[ZM_HP]) {
+#pragma unroll
+    for (int i = 0; i < ZM_PER; i++) {
+        const int h = threadIdx.x + i * ZM_NT;
+        if (h < ZM_HP)
+#pragma unroll
+            for (int k = 0; k < 3; k++) dst[k][h] = r.v[i][k];
     }
 }
 
-__global__ __launch_bounds__(ZM_TX* ZM_TY) void les_tau_fp64_zm(Grid G, Phys P, const double* __restrict__ mac,
-                                                                 const uint64_t* __restrict__ range, int slot,
-                                                                 double* __restrict__ tau, int tiles_x, int tiles_y) {
-    constexpr int HX = ZM_TX + 2, HP = HX * (ZM_TY + 2);
-    __shared__ double su[3][3][HP];  // [ring slot][component][plane node]
+__global__ __launch_bounds__(ZM_NT) void les_tau_fp64_zm(Grid G, Phys P, const double* __restrict__ mac,
+                                                          const uint64_t* __restrict__ range, int slot,
+                                                          double* __restrict__ tau, int tiles_x, int tiles_y) {
+    __shared__ double su[3][3][ZM_HP];  // [ring slot][component][plane node]
     const int64_t N = (int64_t)G.nx * G.ny * G.nz;
     const int x0 = (blockIdx.x % tiles_x) * ZM_TX, y0 = ((blockIdx.x / tiles_x) % tiles_y) * ZM_TY;
     const int zs = (blockIdx.x / (tiles_x * tiles_y)) * ZM_ZC, ze = min(zs + ZM_ZC, G.nz);
@@ -239,10 +262,17 @@ __global__ __launch_bounds__(ZM_TX* ZM_TY) void les_tau_fp64_zm(Grid G, Phys P, 
     const double rmax = from_ordered_bits(range[2 * slot]), rmin = from_ordered_bits(range[2 * slot + 1]);
     const double rg = rmax - rmin;
     // ring slot of plane z' is (z' - zs + 1) mod 3
-    zm_load_plane<double>(G, mac, N, x0, y0, zs - 1, small, su[0]);
-    zm_load_plane<double>(G, mac, N, x0, y0, zs, small, su[1]);
+    zm_store(zm_fetch<double>(G, mac, N, x0, y0, zs - 1, small), su[0]);
+    zm_store(zm_fetch<double>(G, mac, N, x0, y0, zs, small), su[1]);
+    zm_store(zm_fetch<double>(G, mac, N, x0, y0, zs + 1, small), su[2]);
+    const int64_t col = (int64_t)(in ? x : 0) + (int64_t)G.nx * (in ? y : 0);
+    double phi_next = mac[col + (int64_t)G.nx * G.ny * zs];
     for (int z = zs; z < ze; z++) {
-        zm_load_plane<double>(G, mac, N, x0, y0, z + 1, small, su[(z - zs + 2) % 3]);
+        const bool more = z + 1 < ze;  // plane z+2 is needed by plane z+1
+        ZmPlane<double> next;
+        if (more) next = zm_fetch<double>(G, mac, N, x0, y0, z + 2, small);
+        const double phi = phi_next;
+        if (more) phi_next = mac[col + (int64_t)G.nx * G.ny * (z + 1)];
         __syncthreads();
         if (in) {
             const int rz[3] = {(z - zs) % 3, (z - zs + 1) % 3, (z - zs + 2) % 3};  // planes z-1, z, z+1
@@ -252,7 +282,7 @@ __global__ __launch_bounds__(ZM_TX* ZM_TY) void les_tau_fp64_zm(Grid G, Phys P, 
             sfor<1, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
-                const int h = (tx + 1 + D::cx) + HX * (ty + 1 + D::cy);
+                const int h = (tx + 1 + D::cx) + ZM_HX * (ty + 1 + D::cy);
                 for (int k = 0; k < 3; k++) {
                     const double v = su[rz[1 + D::cz]][k][h];
                     gu[k][0] = acc<D::cx>(gu[k][0], v);
@@ -261,7 +291,7 @@ __global__ __launch_bounds__(ZM_TX* ZM_TY) void les_tau_fp64_zm(Grid G, Phys P, 
                 }
             });
             for (int k = 0; k < 3; k++)
-                for (int a = 0; a < 3; a++) gu[k][a] = gu[k][a] / 10.0;
+                for (int a = 0; a < 3; a++) gu[k][a] = div_const(gu[k][a], 10.0, 0.1);
             double sum = 0.0;
             for (int b = 0; b < 3; b++)
                 for (int a = 0; a < 3; a++) {
@@ -269,13 +299,13 @@ __global__ __launch_bounds__(ZM_TX* ZM_TY) void les_tau_fp64_zm(Grid G, Phys P, 
                     sum = sum + sab * sab;
                 }
             const double nue = P.les_c * sqrt(2.0 * sum);
-            const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
-            double frac = (mac[s] - rmin) / rg;
+            double frac = (phi - rmin) / rg;
             if (rg == 0.0) frac = 0.0;
             const double nu = frac * P.dnu + P.nu1;
-            tau[s] = 0.5 + 3.0 * (nu + nue);
+            tau[col + (int64_t)G.nx * G.ny * z] = 0.5 + 3.0 * (nu + nue);
         }
         __syncthreads();
+        if (more) zm_store(next, su[(z - zs) % 3]);  // the slot of plane z-1
     }
 }
 

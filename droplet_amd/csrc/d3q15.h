@@ -45,6 +45,17 @@ template <int Q> __host__ __device__ constexpr double weightF() {
     return Q == 0 ? -7.0 / 3.0 : (Q <= 6 ? 1.0 / 3.0 : 1.0 / 24.0);
 }
 
+// Correctly rounded x / c for c in {3, 5, 10} with r = RN(1/c): q = RN(x r) is
+// faithful, so its remainder e = x - q c is exact in one FMA, and q + e r rounds
+// to RN(x / c) because x / c never lies closer than ulp/20 to a rounding
+// midpoint for these c (a midpoint would need c times a 54-bit odd integer to
+// fit in 53 bits) while the correction is off by at most ulp * 2^-52.
+__device__ __forceinline__ double div_const(double x, double c, double r) {
+    const double q = x * r;
+    const double e = __builtin_fma(-q, c, x);
+    return __builtin_fma(e, r, q);
+}
+
 // signed accumulation helper: s (+|-) v, or s unchanged when c == 0
 template <int C, typename T> __device__ __forceinline__ T acc(T s, T v) {
     if constexpr (C == 1) return s + v;

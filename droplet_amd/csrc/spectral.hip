@@ -36,54 +36,85 @@ constexpr int SF_BLOCK = 256;
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-// e^{s 2 pi i m / n} from the table tw[j] = (cos 2 pi j/n, sin 2 pi j/n), s = +-1
-__device__ __forceinline__ double2 twiddle(const double2* tw, long m, int n, int s) {
-    long j = m % n;
-    if (j < 0) j += n;
+// e^{s 2 pi i m / n} from the table tw[j] = (cos 2 pi j/n, sin 2 pi j/n), s = +-1.
+// Every caller has |m| <= K (n - 1) with K <= 4, so m is reduced into [0, n) by a
+// few compares instead of a (64-bit) integer remainder.
+__device__ __forceinline__ double2 twiddle(const double2* tw, int m, int n, int s) {
+    int j = m;
+    while (j >= n) j -= n;
+    while (j < 0) j += n;
     const double2 t = tw[j];
     return make_double2(t.x, s * t.y);
 }
 
-// pass 1: one wavefront per (y, z) row
+// Sum of NV per-lane values over the 64 lanes of a wavefront with NV - 1 + 1
+// shuffles instead of 6 NV: at each offset o = 32 .. 2 a lane keeps one half of
+// its values (the upper one when lane & o) and adds the partner's copy of that
+// half; after the last halving lane l holds value l >> 1 (summed over its pair's
+// 32 lanes) and one exchange with lane l ^ 1 completes it.  NV <= 32.
+template <int O, int HALF> __device__ __forceinline__ void halve(double (&v)[32], int lane) {
+    const bool up = (lane & O) != 0;
+#pragma unroll
+    for (int i = 0; i < HALF; i++) {
+        const double keep = up ? v[i + HALF] : v[i];
+        const double give = up ? v[i] : v[i + HALF];
+        v[i] = keep + __shfl_xor(give, O, 64);
+    }
+    if constexpr (O > 2) halve<O / 2, HALF / 2>(v, lane);
+}
+
+template <int NV> __device__ __forceinline__ double wave_sums(double (&v)[32], int lane) {
+    static_assert(NV <= 32, "at most 32 values");
+#pragma unroll
+    for (int i = NV; i < 32; i++) v[i] = 0.0;
+    halve<32, 16>(v, lane);
+    return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// pass 1: one wavefront per (y, z) row; the loads of four x-iterations are issued
+// together before their products are accumulated
 template <int K, typename T>
 __global__ __launch_bounds__(SF_BLOCK) void sf_rows_fwd(int nx, int ny, int nz, const T* __restrict__ u1,
                                                         const T* __restrict__ u2, const T* __restrict__ u3,
                                                         const double2* __restrict__ twx, double2* __restrict__ A) {
+    constexpr int NV = 6 * (K + 1), CH = 4;
     const int lane = threadIdx.x & 63;
     const long row = (long)blockIdx.x * (SF_BLOCK / 64) + (threadIdx.x >> 6);
     if (row >= (long)ny * nz) return;  // uniform per wavefront
     const T* u[3] = {u1, u2, u3};
-    double re[3][K + 1], im[3][K + 1];
+    double acc[32];  // [c][k][re, im]
 #pragma unroll
-    for (int c = 0; c < 3; c++)
+    for (int i = 0; i < NV; i++) acc[i] = 0.0;
+    for (int x0 = 0; x0 < nx; x0 += 64 * CH) {
+        double v[CH][3];
+        double2 e[CH][K + 1];
 #pragma unroll
-        for (int k = 0; k <= K; k++) re[c][k] = im[c][k] = 0.0;
-    for (int x = lane; x < nx; x += 64) {
-        const long s = row * nx + x;
-        const double v[3] = {(double)u[0][s], (double)u[1][s], (double)u[2][s]};
+        for (int j = 0; j < CH; j++) {
+            const int x = x0 + 64 * j + lane;
+            const bool ok = x < nx;
+            const long s = row * nx + (ok ? x : 0);
 #pragma unroll
-        for (int k = 0; k <= K; k++) {
-            const double2 e = twiddle(twx, (long)k * x, nx, -1);
+            for (int c = 0; c < 3; c++) v[j][c] = ok ? (double)u[c][s] : 0.0;
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                re[c][k] += v[c] * e.x;
-                im[c][k] += v[c] * e.y;
-            }
+            for (int k = 0; k <= K; k++) e[j][k] = twiddle(twx, k * (ok ? x : 0), nx, -1);
         }
+#pragma unroll
+        for (int j = 0; j < CH; j++)
+#pragma unroll
+            for (int k = 0; k <= K; k++)
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    acc[(c * (K + 1) + k) * 2 + 0] += v[j][c] * e[j][k].x;
+                    acc[(c * (K + 1) + k) * 2 + 1] += v[j][c] * e[j][k].y;
+                }
     }
-#pragma unroll
-    for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int k = 0; k <= K; k++)
-            for (int off = 32; off > 0; off >>= 1) {
-                re[c][k] += __shfl_xor(re[c][k], off, 64);
-                im[c][k] += __shfl_xor(im[c][k], off, 64);
-            }
-    if (lane == 0)
-#pragma unroll
-        for (int c = 0; c < 3; c++)
-#pragma unroll
-            for (int k = 0; k <= K; k++) A[(row * 3 + c) * (K + 1) + k] = make_double2(re[c][k], im[c][k]);
+    const double t = wave_sums<NV>(acc, lane);
+    const int idx = lane >> 1;
+    if ((lane & 1) == 0 && idx < NV) {
+        // idx = (c (K+1) + k) 2 + part
+        double* out = reinterpret_cast<double*>(A + (row * 3) * (K + 1));
+        out[idx] = t;
+    }
 }
 
 // pass 2: one wavefront per (z, c, kx, ky), lanes over y
@@ -98,7 +129,7 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_cols_fwd(int ny, int nz, const do
     const int ky = b % NK - K, kx = (b / NK) % (K + 1), c = b / (NK * (K + 1));
     double sr = 0.0, si = 0.0;
     for (int y = lane; y < ny; y += 64) {
-        const double2 p = cmul(A[(((long)z * ny + y) * 3 + c) * (K + 1) + kx], twiddle(twy, (long)ky * y, ny, -1));
+        const double2 p = cmul(A[(((long)z * ny + y) * 3 + c) * (K + 1) + kx], twiddle(twy, ky * y, ny, -1));
         sr += p.x;
         si += p.y;
     }
@@ -120,7 +151,7 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_zsum(int nx, int ny, int nz, cons
     const int kz = t % NK - K, b = t / NK;
     double sr = 0.0, si = 0.0;
     for (int z = lane; z < nz; z += 64) {
-        const double2 p = cmul(B[(long)z * NB + b], twiddle(twz, (long)kz * z, nz, -1));
+        const double2 p = cmul(B[(long)z * NB + b], twiddle(twz, kz * z, nz, -1));
         sr += p.x;
         si += p.y;
     }
@@ -176,7 +207,7 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_cols_inv(int nz, const double2* _
     const int b = (int)(t % NB), z = (int)(t / NB);
     double2 s = make_double2(0.0, 0.0);
     for (int kz = -K; kz <= K; kz++) {
-        const double2 p = cmul(C[b * NK + (kz + K)], twiddle(twz, (long)kz * z, nz, +1));
+        const double2 p = cmul(C[b * NK + (kz + K)], twiddle(twz, kz * z, nz, +1));
         s.x += p.x;
         s.y += p.y;
     }
@@ -195,38 +226,41 @@ __global__ __launch_bounds__(SF_BLOCK) void sf_rows_inv(int ny, int nz, const do
     const int y = (int)(row % ny), z = (int)(row / ny);
     double2 s = make_double2(0.0, 0.0);
     for (int ky = -K; ky <= K; ky++) {
-        const double2 p = cmul(G[(long)z * NB + ck * NK + (ky + K)], twiddle(twy, (long)ky * y, ny, +1));
+        const double2 p = cmul(G[(long)z * NB + ck * NK + (ky + K)], twiddle(twy, ky * y, ny, +1));
         s.x += p.x;
         s.y += p.y;
     }
     H[t] = s;  // [row][c][kx]
 }
 
-// pass 6: the force field at every node
+// pass 6: the force field, one wavefront per (y, z) row: the row's coefficients are
+// the same for every lane (read once into scalar registers), the lanes run over x
 template <int K>
 __global__ __launch_bounds__(SF_BLOCK) void sf_force(int nx, int ny, int nz, const double2* __restrict__ H,
                                                      const double2* __restrict__ twx, double* __restrict__ fx,
                                                      double* __restrict__ fy, double* __restrict__ fz) {
-    const long n = (long)nx * ny * nz;
-    const long s = blockIdx.x * (long)SF_BLOCK + threadIdx.x;
-    if (s >= n) return;
-    const int x = (int)(s % nx);
-    const long row = s / nx;
-    double f[3];
+    const int lane = threadIdx.x & 63;
+    const long row = (long)blockIdx.x * (SF_BLOCK / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (row >= (long)ny * nz) return;  // uniform per wavefront
+    double2 h[3][K + 1];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const double2* h = H + (row * 3 + c) * (K + 1);
-        double v = h[0].x;
+    for (int c = 0; c < 3; c++)
 #pragma unroll
-        for (int k = 1; k <= K; k++) {
-            const double2 e = twiddle(twx, (long)k * x, nx, +1);
-            v += 2.0 * (h[k].x * e.x - h[k].y * e.y);
+        for (int k = 0; k <= K; k++) h[c][k] = H[(row * 3 + c) * (K + 1) + k];
+    double* const f[3] = {fx, fy, fz};
+    for (int x = lane; x < nx; x += 64) {
+        double2 e[K + 1];
+#pragma unroll
+        for (int k = 1; k <= K; k++) e[k] = twiddle(twx, k * x, nx, +1);
+        const long s = row * nx + x;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double v = h[c][0].x;
+#pragma unroll
+            for (int k = 1; k <= K; k++) v += 2.0 * (h[c][k].x * e[k].x - h[c][k].y * e[k].y);
+            f[c][s] = v;
         }
-        f[c] = v;
     }
-    fx[s] = f[0];
-    fy[s] = f[1];
-    fz[s] = f[2];
 }
 
 template <int K, typename T>
@@ -245,7 +279,7 @@ hipError_t spectral_k(const SpectralWork& w, int nx, int ny, int nz, int k2lo, i
                        w.twz, w.G);
     hipLaunchKernelGGL((sf_rows_inv<K>), dim3((rows * 3 * (K + 1) + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st,
                        ny, nz, w.G, w.twy, w.A);  // H reuses A
-    hipLaunchKernelGGL((sf_force<K>), dim3((n + SF_BLOCK - 1) / SF_BLOCK), dim3(SF_BLOCK), 0, st, nx, ny, nz, w.A,
+    hipLaunchKernelGGL((sf_force<K>), dim3((rows + 3) / 4), dim3(SF_BLOCK), 0, st, nx, ny, nz, w.A,
                        w.twx, fx, fy, fz);
     return hipGetLastError();
 }

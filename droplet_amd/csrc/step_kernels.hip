@@ -50,16 +50,6 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-// Correctly rounded x / c for c in {3, 5, 10} with r = RN(1/c): q = RN(x r) is
-// faithful, so its remainder e = x - q c is exact in one FMA, and q + e r rounds
-// to RN(x / c) because x / c never lies closer than ulp/20 to a rounding
-// midpoint for these c (a midpoint would need c times a 54-bit odd integer to
-// fit in 53 bits) while the correction is off by at most ulp * 2^-52.
-__device__ __forceinline__ double div_const(double x, double c, double r) {
-    const double q = x * r;
-    const double e = __builtin_fma(-q, c, x);
-    return __builtin_fma(e, r, q);
-}
 
 // tau_g: constant; from nu(phi); read from a field (strain Smagorinsky, computed
 // after each step); from nu(phi) and the stored sqrt(Pi:Pi) of the node's
