@@ -18,14 +18,14 @@ __global__ __launch_bounds__(256) void stream_copy(const double2* __restrict__ s
 // K independent SoA streams copied at once, x fastest with one double per
 // lane, non-temporal stores: the access pattern of the step kernel (15 + 5
 // arrays in, 15 + 5 out) without its arithmetic
-template <int K>
+template <int K, bool NTL = false>
 __global__ __launch_bounds__(256) void stream_copy_soa(const double* __restrict__ src, double* __restrict__ dst,
                                                        int64_t n) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
         double v[K];
 #pragma unroll
-        for (int k = 0; k < K; k++) v[k] = src[k * n + i];
+        for (int k = 0; k < K; k++) v[k] = NTL ? __builtin_nontemporal_load(&src[k * n + i]) : src[k * n + i];
 #pragma unroll
         for (int k = 0; k < K; k++) __builtin_nontemporal_store(v[k], &dst[k * n + i]);
     }
@@ -89,8 +89,8 @@ __global__ __launch_bounds__(256) void stream_write(dvec2* __restrict__ dst, int
 }  // namespace
 
 extern "C" int drp_copy_bandwidth_kind(size_t bytes, int kind, int reps, double* gbs) {
-    if (!gbs || bytes < 4096 || reps < 1 || kind < 0 || kind > 5)
-        return drp_fail(DRP_ERR_ARG, "bad probe arguments (kind 0..5)");
+    if (!gbs || bytes < 4096 || reps < 1 || kind < 0 || kind > 6)
+        return drp_fail(DRP_ERR_ARG, "bad probe arguments (kind 0..6)");
     bytes &= ~(size_t)15;
     void *a = nullptr, *b = nullptr;
     hipStream_t st = nullptr;
@@ -118,11 +118,16 @@ extern "C" int drp_copy_bandwidth_kind(size_t bytes, int kind, int reps, double*
             case 4:
                 hipLaunchKernelGGL((stream_read<U>), dim3(blocks_u), dim3(256), 0, st, (const dvec2*)a, (double*)b, n2);
                 break;
+            case 6:  // 20 structure-of-arrays streams of bytes / 20, one double per lane
+                hipLaunchKernelGGL((stream_copy_soa<20, true>), dim3(256 * 8 * 4), dim3(256), 0, st, (const double*)a,
+                                   (double*)b, (int64_t)(bytes / 160));
+                break;
             default: hipLaunchKernelGGL((stream_write<U>), dim3(blocks_u), dim3(256), 0, st, (dvec2*)b, n2); break;
         }
         return hipGetLastError();
     };
-    if (kind >= 4) moved = (double)bytes;
+    if (kind == 4 || kind == 5) moved = (double)bytes;
+    if (kind == 6) moved = 2.0 * (double)(bytes / 160 * 160);
     // the read-only kernel's sink (never written for a zero source) spans
     // blocks_u * 256 doubles <= bytes / 8 bytes, inside b
     if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {

@@ -336,7 +336,8 @@ def batch_autotune(cfgs, nsteps: int = 10):
 
 def copy_bandwidth_kind(nbytes: int, kind: int, reps: int = 20) -> float:
     """GB/s of yardstick `kind` (0 grid-stride copy, 1 hipMemcpyDtoD, 2 unrolled non-temporal
-    copy, 3 unrolled plain copy, 4 read only, 5 write only) over nbytes."""
+    copy, 3 unrolled plain copy, 4 read only, 5 write only, 6 twenty SoA streams with
+    non-temporal loads) over nbytes."""
     gbs = C.c_double()
     _check(lib().drp_copy_bandwidth_kind(int(nbytes), int(kind), int(reps), C.byref(gbs)), "drp_copy_bandwidth_kind")
     return gbs.value

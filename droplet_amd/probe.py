@@ -8,7 +8,7 @@
     python -m droplet_amd.probe yardsticks [--reps 20]
         independent bandwidth ceilings at 1, 2 and 4 GiB: grid-stride copy,
         hipMemcpyDtoD, 4x-unrolled non-temporal and plain copies, read only, write
-        only; then the copysoa streams at 256^3
+        only, 20 SoA streams with non-temporal loads; then the copysoa streams at 256^3
     python -m droplet_amd.probe step [--grid 256] [--steps 20] [--fp32] [--shape S]
         the 256^3 shear-droplet step kernel of bench.py, a few launches
     python -m droplet_amd.probe hit [--grid 256] [--steps 20] [--fp32] [--we 5]
@@ -22,6 +22,9 @@
     python -m droplet_amd.probe spectra [--grid 256] [--reps 3]
         FFT diagnostics of the 256^3 turbulent-droplet state: energy spectrum,
         per-scale interaction and a low-pass filter, milliseconds per call
+    python -m droplet_amd.probe adjoint [--grid 256] [--steps 20] [--checkpoint 10]
+        taped FP64 run of the shear droplet and the checkpointed adjoint sweep of
+        J = D + A: ms per forward and per reverse step
     python -m droplet_amd.probe ensemble [--members 64] [--steps 60] [--fp64]
         64 shear-droplet members, N uniform in {48, 64, 96, 128}, radius N/5,
         Ca uniform in [0.05, 0.5], centre offsets within +-2 cells
@@ -155,6 +158,33 @@ def spectra(a, dl):
     print(json.dumps(out))
 
 
+def adjoint(a, dl):
+    """Taped FP64 run of bench.py's shear droplet (checkpoint every K steps) and
+    the checkpointed adjoint sweep of J = D + A: ms per taped forward step and per
+    reverse step (segment recomputation, adjoint collision and stencil kernels)."""
+    import time
+    from bench import shear_config
+    cfg, drops, _ = shear_config(a.grid)
+    lat = dl.Lattice(cfg)
+    lat.init_droplets(drops, 1.0 / 3.0)
+    tape = dl.Tape(lat, a.checkpoint)
+    t0 = time.perf_counter()
+    tape.step(a.steps)
+    J, D, A = tape.loss()
+    t1 = time.perf_counter()
+    J2, g = tape.backward((1.0, 1.0))
+    t2 = time.perf_counter()
+    n = a.grid ** 3
+    fwd, rev = (t1 - t0) * 1e3 / a.steps, (t2 - t1) * 1e3 / a.steps
+    print(json.dumps({"workload": f"adjoint of the shear droplet {a.grid}^3", "steps": a.steps,
+                      "checkpoint_every": a.checkpoint, "tape_GB": round(tape.nbytes / 1e9, 2),
+                      "forward_ms_per_step": round(fwd, 3), "reverse_ms_per_step": round(rev, 3),
+                      "reverse_MLUPS": round(n / (rev * 1e-3) / 1e6, 1), "J": J, "D": D, "A": A,
+                      "dJ": g}))
+    tape.close()
+    lat.close()
+
+
 def droplet_census(a, dl):
     """Census of the turbulent-droplet state (32 droplets of radius 10) after a
     few steps: droplet count, volume range and time per call (HIP sync included)."""
@@ -180,7 +210,7 @@ def droplet_census(a, dl):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["copy", "copysoa", "yardsticks", "step", "autotune", "ensemble", "hit", "census", "allen-cahn",
-                                     "spectra"])
+                                     "spectra", "adjoint"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
@@ -192,6 +222,7 @@ def main(argv=None):
                     help="ensemble member grid sizes to draw from (comma separated)")
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--we", type=float, default=5.0)
+    ap.add_argument("--checkpoint", type=int, default=10, help="adjoint: checkpoint every K steps")
     a = ap.parse_args(argv)
     from droplet_amd import lattice as dl
     if a.what == "ensemble":
@@ -209,12 +240,15 @@ def main(argv=None):
     if a.what == "spectra":
         spectra(a, dl)
         return
+    if a.what == "adjoint":
+        adjoint(a, dl)
+        return
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
         print(json.dumps({"copy_bytes": a.mib << 20, "reps": a.reps, "GB/s": round(gbs, 1)}))
         return
     if a.what == "yardsticks":
-        names = ["gridstride", "memcpyDtoD", "unroll4_nt", "unroll4_plain", "read_only", "write_only"]
+        names = ["gridstride", "memcpyDtoD", "unroll4_nt", "unroll4_plain", "read_only", "write_only", "soa20_nt"]
         for mib in (1024, 2048, 4096):
             out = {"MiB": mib}
             for k, name in enumerate(names):

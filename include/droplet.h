@@ -344,7 +344,9 @@ int drp_copy_bandwidth(size_t bytes, int reps, double* gbs);
 /* Independent yardsticks for the same `bytes`: kind 0 the grid-stride copy above,
  * 1 hipMemcpyDtoDAsync, 2 a one-pass copy with four 16-byte non-temporal loads
  * in flight per lane before its stores, 3 the same with ordinary loads and
- * stores, 4 read only (bytes read), 5 write only (bytes written). */
+ * stores, 4 read only (bytes read), 5 write only (bytes written), 6 twenty
+ * structure-of-arrays streams of bytes/20 copied at once, one double per lane,
+ * non-temporal loads and stores (the step kernel's access pattern). */
 int drp_copy_bandwidth_kind(size_t bytes, int kind, int reps, double* gbs);
 
 /* The same for `streams` (10, 20 or 40) structure-of-arrays streams of
