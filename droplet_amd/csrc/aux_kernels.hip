@@ -96,8 +96,8 @@ __global__ void init_equilibrium_fp64(Grid G, Phys P, const double* __restrict__
     sfor<0, 15>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const double gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
-        f[q * N + s] = feq<q>(nd, gt, P.kf);
-        g[q * N + s] = geq<q>(nd, usq, gt, P.kg);
+        f[q * N + s] = feq<q>(nd, gt, ekappa<q>(P.kf));
+        g[q * N + s] = geq<q>(nd, usq, gt, ekappa<q>(P.kg));
     });
 }
 

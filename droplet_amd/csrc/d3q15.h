@@ -56,6 +56,41 @@ __device__ __forceinline__ double div_const(double x, double c, double r) {
     return __builtin_fma(e, r, q);
 }
 
+// x / d for a divisor d that many divisions share (a constant relaxation time).
+// gfx950 divides doubles by the sequence: y = rcp(d) refined by two Newton
+// steps, q = x y, r = fma(-d, q, x), q + r y, then a sign / special-value fixup,
+// with range scalings of x and d that apply only when |x| < 2^-968, the exponents
+// of x and d differ by 768 or more, or x / d or 1 / d is subnormal.  The steps on
+// d alone are done once (SharedDivisor).  For 1/4 <= d <= 2^32 and x = +0 or
+// 2^-967 <= |x| < 2^766 none of the scalings applies and the fixup returns q
+// unchanged, so the three remaining steps (div_shared_fast) give exactly x / d;
+// div_shared_exact tells whether x is such a value.  Callers test every x and
+// redo the divisions of a node with the full division when one is not.
+struct SharedDivisor {
+    double d, y;
+    bool ok;  // 1/4 <= d <= 2^32
+};
+__device__ __forceinline__ SharedDivisor shared_divisor(double d) {
+    SharedDivisor s;
+    s.d = d;
+    double y = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-d, y, 1.0);
+    s.y = __builtin_fma(y, e, y);
+    s.ok = d >= 0.25 && d <= 0x1p32;
+    return s;
+}
+__device__ __forceinline__ double div_shared_fast(double x, const SharedDivisor& s) {
+    const double q = x * s.y;
+    const double r = __builtin_fma(-s.d, q, x);
+    return __builtin_fma(r, s.y, q);
+}
+__device__ __forceinline__ bool div_shared_exact(double x) {
+    const double ax = __builtin_fabs(x);
+    return (ax >= 0x1p-967 && ax < 0x1p766) || __builtin_amdgcn_class(x, 0x040 /* +0 */);
+}
+
 // signed accumulation helper: s (+|-) v, or s unchanged when c == 0
 template <int C, typename T> __device__ __forceinline__ T acc(T s, T v) {
     if constexpr (C == 1) return s + v;
@@ -119,20 +154,26 @@ template <typename T> struct NodeEq {
     T tau;        // momentum relaxation time tau_g
 };
 
-// feq_q (:452-459): ((H phi + F A) + ((E phi) 3) cu) + (E kf) gt
-template <int Q, typename T> __device__ __forceinline__ T feq(const NodeEq<T>& n, T gt, T kf) {
+// weight class of direction q (0 rest, 1 axis, 2 diagonal) and E_q kappa, the
+// factor of gt in the equilibria: the reference forms E(i) * kappa first, so
+// callers may pass that product computed once per class
+template <int Q> __host__ __device__ constexpr int wclass() { return Q == 0 ? 0 : (Q <= 6 ? 1 : 2); }
+template <int Q, typename T> __host__ __device__ __forceinline__ T ekappa(T k) { return T(weightE<Q>()) * k; }
+
+// feq_q (:452-459): ((H phi + F A) + ((E phi) 3) cu) + (E kf) gt, ekf = E kf
+template <int Q, typename T> __device__ __forceinline__ T feq(const NodeEq<T>& n, T gt, T ekf) {
     constexpr T E = T(weightE<Q>()), Fw = T(weightF<Q>());
     T s = (Q == 0) ? (n.phi + Fw * n.A) : (Fw * n.A);
     if constexpr (Q != 0) {
         T cu = cdot<Q>(n.u1, n.u2, n.u3);
         s = s + E * n.phi * T(3.0) * cu;
-        s = s + E * kf * gt;
+        s = s + ekf * gt;
     }
     return s;
 }
 
-// geq_q (:461-469): E (((3p + 3cu) + 4.5 cu^2) - 1.5 |u|^2) + (E kg) gt   (Anu = 0)
-template <int Q, typename T> __device__ __forceinline__ T geq(const NodeEq<T>& n, T usq, T gt, T kg) {
+// geq_q (:461-469): E (((3p + 3cu) + 4.5 cu^2) - 1.5 |u|^2) + (E kg) gt   (Anu = 0), ekg = E kg
+template <int Q, typename T> __device__ __forceinline__ T geq(const NodeEq<T>& n, T usq, T gt, T ekg) {
     constexpr T E = T(weightE<Q>());
     if constexpr (Q == 0) {
         return E * (T(3.0) * n.p - T(1.5) * usq);
@@ -141,7 +182,7 @@ template <int Q, typename T> __device__ __forceinline__ T geq(const NodeEq<T>& n
         T s = T(3.0) * n.p + T(3.0) * cu;
         s = s + T(4.5) * (cu * cu);
         s = s - T(1.5) * usq;
-        return E * s + E * kg * gt;
+        return E * s + ekg * gt;
     }
 }
 

@@ -57,6 +57,7 @@ struct StepArgs {
     const double *sinx, *cosx, *sinz, *cosz;  // FORCE_TAYLOR_GREEN tables
     double tau_f, kf, kf6, kg, T, a, b, nu1, dnu, tg_amp, phi1, phi2;
     double tau_const;     // tau_g when nu1 == nu2: 0.5 + 3 nu1
+    double ekf[3], ekg[3];  // E kappa_f, E kappa_g for the rest, axis and diagonal weights
     double wall_u_now;
     int wall_axis;        // 0, 2 (y) or 3 (z)
     int force_kind;

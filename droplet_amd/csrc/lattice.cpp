@@ -90,6 +90,8 @@ static int validate(const drp_config* c) {
     if (!c) return drp_fail(DRP_ERR_ARG, "null config");
     if (c->nx < 3 || c->ny < 3 || c->nz < 3) return drp_fail(DRP_ERR_ARG, "grid must be at least 3^3");
     if ((int64_t)c->nx * c->ny * c->nz * 15 > (int64_t)1 << 40) return drp_fail(DRP_ERR_ARG, "grid too large");
+    if ((int64_t)c->nx * c->ny >= (int64_t)1 << 31)
+        return drp_fail(DRP_ERR_ARG, "an x-y plane of 2^31 or more nodes is not supported");
     if (c->precision != DRP_FP64 && c->precision != DRP_FP32_SHIFTED)
         return drp_fail(DRP_ERR_ARG, "unknown precision %d", c->precision);
     if (!(c->tau_f > 0.5)) return drp_fail(DRP_ERR_ARG, "tau_f must exceed 1/2");
@@ -456,6 +458,12 @@ StepArgs make_step_args(drp_lattice* L, int64_t nstep) {
     a.nu1 = P.nu1;
     a.dnu = P.dnu;
     a.tau_const = 0.5 + 3.0 * P.nu1;
+    a.ekf[0] = ekappa<0>(P.kf);
+    a.ekf[1] = ekappa<1>(P.kf);
+    a.ekf[2] = ekappa<7>(P.kf);
+    a.ekg[0] = ekappa<0>(P.kg);
+    a.ekg[1] = ekappa<1>(P.kg);
+    a.ekg[2] = ekappa<7>(P.kg);
     a.tg_amp = P.tg_amp;
     a.phi1 = P.phi1;
     a.phi2 = P.phi2;

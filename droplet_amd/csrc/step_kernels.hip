@@ -81,6 +81,10 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     constexpr int HX2 = TX + 4, HY2 = TY + 4, HZ2 = TZ + 4, H2 = HX2 * HY2 * HZ2;
     constexpr int HX1 = TX + 2, HY1 = TY + 2, HZ1 = TZ + 2, H1 = HX1 * HY1 * HZ1;
     static_assert(NN % 64 == 0, "a role must own whole wavefronts");
+    // relaxation by the shared-divisor division (d3q15.h) where it costs no
+    // spilled registers: not with the register rings of a force field or of the
+    // band-passed velocity
+    constexpr bool FASTDIV = FORCE != FORCE_FIELD && MODEL != MODEL_SCALE;
 
     __shared__ double s_phi[H2];
     __shared__ double s_p[H1], s_u1[H1], s_u2[H1], s_u3[H1];
@@ -110,8 +114,10 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const int z0 = ((tile / a.tiles_x) % a.tiles_z) * TZ;
     const int y0 = (tile / (a.tiles_x * a.tiles_z)) * TY;
     const int tid = threadIdx.x;
-    const bool mom = tid >= NN;          // role: momentum populations (uniform per wavefront)
+    // role: momentum populations; uniform per wavefront (NN is a multiple of 64)
+    const bool mom = tid >= NN;
     const int nid = mom ? tid - NN : tid;
+    __builtin_assume(nid >= 0 && nid < NN);
 
     const double* __restrict__ phi_in = a.mac_in;
     const double* __restrict__ p_in = a.mac_in + N;
@@ -120,25 +126,41 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const double* __restrict__ u3_in = a.mac_in + 4 * N;
 
     // ---------------- phase 1a: load the macroscopic fields of the tile and its
-    // halos into registers
+    // halos into registers.  Thread t stages column t % HX of a halo box, in
+    // rows t / HX + k (NT / HX) (row = y + HY z): its x coordinate and wrap are
+    // fixed, and the rows of a tile whose halo box lies inside the grid in y and
+    // z (every tile but those of the first and last layers) need no wrap
     const bool wm = a.wrap_mod != 0;
-    constexpr int IT2 = (H2 + NT - 1) / NT, IT1 = (H1 + NT - 1) / NT;
+    constexpr int RP2 = NT / HX2, R2 = HY2 * HZ2, IT2 = (R2 + RP2 - 1) / RP2;
+    constexpr int RP1 = NT / HX1, R1 = HY1 * HZ1, IT1 = (R1 + RP1 - 1) / RP1;
+    static_assert(RP2 >= 1 && RP1 >= 1, "a workgroup spans at least one halo row");
+    const int c2 = tid % HX2, r2 = tid / HX2;  // r2 < RP2: the thread stages phi rows
+    const int c1 = tid % HX1, r1 = tid / HX1;  // r1 < RP1: the thread stages p, u rows
+    const bool yz_in = !wm && y0 >= 2 && y0 + TY + 2 <= ny && z0 >= 2 && z0 + TZ + 2 <= nz;
+    // node index of (0, y, z): z nx ny in 64 bits from 32-bit factors (an x-y
+    // plane holds fewer than 2^31 nodes)
+    const uint32_t nxy32 = (uint32_t)nx * (uint32_t)ny;
+    auto row_offset = [&](int hy, int hz, int halo) -> int64_t {
+        int gy = y0 - halo + hy, gz = z0 - halo + hz;
+        if (!yz_in) {
+            gy = wrapi(gy, ny, wm);
+            gz = wrapi(gz, nz, wm);
+        }
+        return (int64_t)((uint64_t)(uint32_t)gz * nxy32 + (uint32_t)(gy * nx));
+    };
+    const int gx2 = wrapi(x0 - 2 + c2, nx, wm), gx1 = wrapi(x0 - 1 + c1, nx, wm);
     double vphi[IT2], vp[IT1], vu1[IT1], vu2[IT1], vu3[IT1], vtau[IT1];
     static_for<0, IT2>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        // loads are unconditional (indices clamped into the halo box) so that
-        // the compiler's wait for them is a straight-line count
-        const int h = (tid + k * NT < H2) ? tid + k * NT : H2 - 1;
-        const int hx = h % HX2, hy = (h / HX2) % HY2, hz = h / (HX2 * HY2);
-        const int gx = wrapi(x0 + hx - 2, nx, wm), gy = wrapi(y0 + hy - 2, ny, wm), gz = wrapi(z0 + hz - 2, nz, wm);
-        vphi[k] = phi_in[gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
+        // loads are unconditional (rows clamped into the halo box) so that the
+        // compiler's wait for them is a straight-line count
+        const int r = r2 + k * RP2 < R2 ? r2 + k * RP2 : R2 - 1;
+        vphi[k] = phi_in[gx2 + row_offset(r % HY2, r / HY2, 2)];
     });
     static_for<0, IT1>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int h = (tid + k * NT < H1) ? tid + k * NT : H1 - 1;
-        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
-        const int gx = wrapi(x0 + hx - 1, nx, wm), gy = wrapi(y0 + hy - 1, ny, wm), gz = wrapi(z0 + hz - 1, nz, wm);
-        const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
+        const int r = r1 + k * RP1 < R1 ? r1 + k * RP1 : R1 - 1;
+        const int64_t g = gx1 + row_offset(r % HY1, r / HY1, 1);
         vp[k] = p_in[g];
         vu1[k] = u1_in[g];
         vu2[k] = u2_in[g];
@@ -152,38 +174,52 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     // not include them and their latency overlaps staging and stencils; they
     // are read once, non-temporally, to leave L2 to the macroscopic halos
     const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
+    // LDS indices of the corner (x - 1, y - 1, z - 1) in the 1-halo arrays and of
+    // (x - 2, y - 2, z - 2) in the 2-halo phi array: the upstream node of every
+    // direction lies at a non-negative constant offset from them, which the LDS
+    // reads take as their immediate offset
+    const int hb1 = tx + HX1 * (ty + HY1 * tz);
+    const int hb2 = tx + HX2 * (ty + HY2 * tz);
     const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
     const bool inside = (x < nx) && (y < ny) && (z < nz);
     // upstream coordinate x - c is entry [c + 1] of {x+1, x, x-1} (periodic wrap)
     const int xc = inside ? x : 0, yc = inside ? y : 0, zc = inside ? z : 0;
     const int xs[3] = {xc + 1 == nx ? 0 : xc + 1, xc, xc == 0 ? nx - 1 : xc - 1};
     const int zs[3] = {zc + 1 == nz ? 0 : zc + 1, zc, zc == 0 ? nz - 1 : zc - 1};
-    const int64_t yo[3] = {(int64_t)nx * (yc + 1 == ny ? 0 : yc + 1), (int64_t)nx * yc,
-                           (int64_t)nx * (yc == 0 ? ny - 1 : yc - 1)};
-    const int64_t nxy = (int64_t)nx * ny;
-    const int64_t zo[3] = {nxy * zs[0], nxy * zs[1], nxy * zs[2]};
-    const double* __restrict__ pin = mom ? a.g_in : a.f_in;
+    const uint32_t yo[3] = {(uint32_t)(nx * (yc + 1 == ny ? 0 : yc + 1)), (uint32_t)(nx * yc),
+                            (uint32_t)(nx * (yc == 0 ? ny - 1 : yc - 1))};
+    const uint64_t zo[3] = {(uint64_t)(uint32_t)zs[0] * nxy32, (uint64_t)(uint32_t)zs[1] * nxy32,
+                            (uint64_t)(uint32_t)zs[2] * nxy32};
+    // node index of the upstream node x - c of direction q
+    auto upstream = [&](auto qc) -> int64_t {
+        using D = Dir<decltype(qc)::value>;
+        return (int64_t)(zo[1 + D::cz] + (uint64_t)(yo[1 + D::cy] + (uint32_t)xs[1 + D::cx]));
+    };
+    // the role's population array, as a scalar (the role is uniform per wavefront)
+    const double* __restrict__ pin = (__builtin_amdgcn_readfirstlane(mom ? 1 : 0) != 0) ? a.g_in : a.f_in;
     // unconditional: threads outside the grid read node (0, 0, 0)'s neighbours
     // (the Allen-Cahn model has no phi populations to load)
     double ps[15];
-    if (MODEL != MODEL_AC || mom) {
+    auto load_pops = [&]() {
         static_for<0, 15>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             using D = Dir<q>;
-            ps[q] = __builtin_nontemporal_load(&pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]]);
+            ps[q] = __builtin_nontemporal_load(&pin[q * N + upstream(qc)]);
         });
-    }
+    };
+    if (MODEL != MODEL_AC || mom) load_pops();
 
     // ---------------- phase 1c: stage the fields in LDS
     static_for<0, IT2>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int h = tid + k * NT;
-        if (h < H2) s_phi[h] = vphi[k];
+        const int r = r2 + k * RP2;
+        if (r2 < RP2 && r < R2) s_phi[c2 + HX2 * r] = vphi[k];
     });
     static_for<0, IT1>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int h = tid + k * NT;
-        if (h < H1) {
+        const int r = r1 + k * RP1;
+        if (r1 < RP1 && r < R1) {
+            const int h = c1 + HX1 * r;
             s_p[h] = vp[k];
             s_u1[h] = vu1[k];
             s_u2[h] = vu2[k];
@@ -197,19 +233,21 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     const double rmax = from_ordered_bits(a.range[2 * a.slot_read + 0]);
     const double rmin = from_ordered_bits(a.range[2 * a.slot_read + 1]);
     const double range = rmax - rmin;
-    static_for<0, (H1 + NT - 1) / NT>([&](auto kc) {
+    static_for<0, IT1>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int h = tid + k * NT;
-        if (h >= H1) return;
-        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
-        const int c = (hx + 1) + HX2 * ((hy + 1) + HY2 * (hz + 1));
-        const double ph = s_phi[c];
+        const int r = r1 + k * RP1;
+        if (r1 >= RP1 || r >= R1) return;
+        const int h = c1 + HX1 * r;
+        // phi-box index of the node's (-1, -1, -1) neighbour: the stencil reads lie
+        // at non-negative constant offsets from it
+        const int cb = c1 + HX2 * (r % HY1 + HY2 * (r / HY1));
+        const double ph = s_phi[cb + (1 + HX2 * (1 + HY2))];
         double lap = -14.0 * ph;
         double g0 = 0.0, g1 = 0.0, g2 = 0.0;
         static_for<1, 15>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             using D = Dir<q>;
-            const double v = s_phi[c + D::cx + HX2 * (D::cy + HY2 * D::cz)];
+            const double v = s_phi[cb + ((1 + D::cx) + HX2 * ((1 + D::cy) + HY2 * (1 + D::cz)))];
             lap = lap + v;
             g0 = acc<D::cx>(g0, v);
             g1 = acc<D::cy>(g1, v);
@@ -268,8 +306,8 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
                 constexpr double E = weightE<q>();
-                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
-                const int h2 = (tx + 2 - D::cx) + HX2 * ((ty + 2 - D::cy) + HY2 * (tz + 2 - D::cz));
+                const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
+                const int h2 = hb2 + ((2 - D::cx) + HX2 * ((2 - D::cy) + HY2 * (2 - D::cz)));
                 const double phu = s_phi[h2];
                 double fe = E * phu;
                 if constexpr (q != 0) {
@@ -284,6 +322,11 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             vmax = ph;
             vmin = ph;
         } else if (!mom) {
+            // relaxation by 1/tau_f with the divisor's steps done once (SharedDivisor);
+            // a node one of whose f - feq lies outside the range where that is exact
+            // runs its collision again with full divisions
+            const SharedDivisor dtau = shared_divisor(a.tau_f);
+            bool redo = !dtau.ok;
             double ph = 0.0;
             // MODEL_SCALE: the band-passed velocity of the upstream nodes (equilibrium_cal,
             // program/lbm_drop_scale_xy.f90:537-540), loaded from L2 UPF directions ahead
@@ -293,19 +336,21 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             auto load_uadv = [&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
-                const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
+                const int64_t src = upstream(qc);
                 double* u = uring[q % UPF];
                 if constexpr (D::cx != 0) u[0] = a.uadv[src];
                 if constexpr (D::cy != 0) u[1] = a.uadv[N + src];
                 if constexpr (D::cz != 0) u[2] = a.uadv[2 * N + src];
             };
+            auto collide = [&](auto full_c) {
+            constexpr bool FULL = decltype(full_c)::value;
             if constexpr (MODEL == MODEL_SCALE)
                 static_for<1, 1 + UPF>([&](auto qc) { load_uadv(qc); });
             static_for<0, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
-                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
-                const int h2 = (tx + 2 - D::cx) + HX2 * ((ty + 2 - D::cy) + HY2 * (tz + 2 - D::cz));
+                const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
+                const int h2 = hb2 + ((2 - D::cx) + HX2 * ((2 - D::cy) + HY2 * (2 - D::cz)));
                 NodeEq<double> nd;
                 nd.phi = s_phi[h2];
                 nd.A = s_A[hu];
@@ -329,12 +374,26 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                     const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
                     gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
                 }
-                const double fe = feq<q>(nd, gt, a.kf);
-                const double fv = ps[q] - (ps[q] - fe) / a.tau_f;
+                const double fe = feq<q>(nd, gt, a.ekf[wclass<q>()]);
+                const double dx = ps[q] - fe;
+                double fv;
+                if constexpr (FULL) {
+                    fv = ps[q] - dx / a.tau_f;
+                } else {
+                    fv = ps[q] - div_shared_fast(dx, dtau);
+                    redo |= !div_shared_exact(dx);
+                }
                 ph = (q == 0) ? fv : ph + fv;
                 __builtin_nontemporal_store(fv, &a.f_out[q * N + s]);
                 __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
             });
+            };
+            if constexpr (FASTDIV) {
+                collide(std::false_type{});
+                if (redo) collide(std::true_type{});
+            } else {
+                collide(std::true_type{});
+            }
             __builtin_nontemporal_store(ph, &a.mac_out[s]);
             vmax = ph;
             vmin = ph;
@@ -347,17 +406,23 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             auto load_force = [&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
-                const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
+                const int64_t src = upstream(qc);
                 double* f = fring[q % FPF];
                 if constexpr (D::cx != 0) f[0] = a.fx[src];
                 if constexpr (D::cy != 0) f[1] = a.fy[src];
                 if constexpr (D::cz != 0) f[2] = a.fz[src];
             };
+            // with a constant tau_g, relaxation as in the phi role (SharedDivisor, and
+            // the collision again with full divisions for a node that needs them)
+            const SharedDivisor dtau = shared_divisor(a.tau_const);
+            bool redo = TAUMODE == TAU_CONST && !dtau.ok;
+            auto collide = [&](auto full_c) {
+            constexpr bool FULL = decltype(full_c)::value;
             if constexpr (FORCE == FORCE_FIELD) static_for<1, 1 + FPF>(load_force);
             static_for<0, 15>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
-                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
                 NodeEq<double> nd;
                 nd.p = s_p[hu];
                 nd.u1 = s_u1[hu];
@@ -372,9 +437,17 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                     const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
                     gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
                 }
-                const double ge = geq<q>(nd, usq, gt, a.kg);
-                const double tau = (TAUMODE == TAU_CONST) ? a.tau_const : s_tau[hu];
-                double gv = ps[q] - (ps[q] - ge) / tau;
+                const double ge = geq<q>(nd, usq, gt, a.ekg[wclass<q>()]);
+                const double dx = ps[q] - ge;
+                double gv;
+                if constexpr (TAUMODE != TAU_CONST) {
+                    gv = ps[q] - dx / s_tau[hu];
+                } else if constexpr (FULL) {
+                    gv = ps[q] - dx / a.tau_const;
+                } else {
+                    gv = ps[q] - div_shared_fast(dx, dtau);
+                    redo |= !div_shared_exact(dx);
+                }
                 if constexpr (FORCE != FORCE_NONE && q != 0) {
                     // c . F with the zero-velocity terms dropped, as for c . u: the
                     // reference's (cx Fx + cy Fy) + cz Fz rounds identically, and
@@ -397,6 +470,23 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 ps[q] = gv;
                 __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
             });
+            };
+            collide(std::integral_constant<bool, !FASTDIV>{});
+            if constexpr (FASTDIV && TAUMODE == TAU_CONST) {
+                if (redo) {
+                    // the populations this step started from, again, addressed from
+                    // the node's coordinates (so that phase 1's upstream offsets
+                    // need not stay in registers until here)
+                    static_for<0, 15>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        using D = Dir<q>;
+                        const int ux = wrapi(x - D::cx, nx, false), uy = wrapi(y - D::cy, ny, false),
+                                  uz = wrapi(z - D::cz, nz, false);
+                        ps[q] = __builtin_nontemporal_load(&a.g_in[q * N + ux + (int64_t)nx * (uy + (int64_t)ny * uz)]);
+                    });
+                    collide(std::true_type{});
+                }
+            }
 
             if constexpr (WALL) {
                 // bounce_back_LBM, dropch.f90:666-698: on the first/last fluid layer
@@ -439,7 +529,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                     constexpr int q = decltype(qc)::value;
                     using D = Dir<q>;
                     constexpr int c[3] = {D::cx, D::cy, D::cz};
-                    const double d = ps[q] - geq<q>(nd, usq, korteweg<q>(g0, g1, g2, s2h), a.kg);
+                    const double d = ps[q] - geq<q>(nd, usq, korteweg<q>(g0, g1, g2, s2h), a.ekg[wclass<q>()]);
 #pragma unroll
                     for (int b = 0; b < 3; b++)
 #pragma unroll
