@@ -172,18 +172,23 @@ template <int Q, typename T> __device__ __forceinline__ T feq(const NodeEq<T>& n
     return s;
 }
 
-// geq_q (:461-469): E (((3p + 3cu) + 4.5 cu^2) - 1.5 |u|^2) + (E kg) gt   (Anu = 0), ekg = E kg
-template <int Q, typename T> __device__ __forceinline__ T geq(const NodeEq<T>& n, T usq, T gt, T ekg) {
+// geq_q (:461-469): E (((3p + 3cu) + 4.5 cu^2) - 1.5 |u|^2) + (E kg) gt   (Anu = 0), ekg = E kg,
+// from the node's 3p and 1.5 |u|^2 (per-node products a caller may form once)
+template <int Q, typename T>
+__device__ __forceinline__ T geq_pre(T p3, T u1, T u2, T u3, T usq15, T gt, T ekg) {
     constexpr T E = T(weightE<Q>());
     if constexpr (Q == 0) {
-        return E * (T(3.0) * n.p - T(1.5) * usq);
+        return E * (p3 - usq15);
     } else {
-        T cu = cdot<Q>(n.u1, n.u2, n.u3);
-        T s = T(3.0) * n.p + T(3.0) * cu;
+        T cu = cdot<Q>(u1, u2, u3);
+        T s = p3 + T(3.0) * cu;
         s = s + T(4.5) * (cu * cu);
-        s = s - T(1.5) * usq;
+        s = s - usq15;
         return E * s + ekg * gt;
     }
+}
+template <int Q, typename T> __device__ __forceinline__ T geq(const NodeEq<T>& n, T usq, T gt, T ekg) {
+    return geq_pre<Q>(T(3.0) * n.p, n.u1, n.u2, n.u3, T(1.5) * usq, gt, ekg);
 }
 
 // order-preserving map of doubles to uint64 (for atomic min/max of the phi range)

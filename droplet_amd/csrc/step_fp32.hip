@@ -70,7 +70,11 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     // the variable-tau variants at two workgroups per CU (an own array would add
     // 4 B per halo node and exceed half of the LDS)
     float* const s_rtau = s_phi;
-    constexpr int IT1 = (H1 + NT - 1) / NT;
+    // staging: thread t owns column t % HX of a halo box, rows t / HX + k (NT / HX)
+    // (row = y + HY z), so that its x coordinate and wrap are fixed
+    constexpr int RP2 = NT / HX2, R2 = HY2 * HZ2, IT2 = (R2 + RP2 - 1) / RP2;
+    constexpr int RP1 = NT / HX1, R1 = HY1 * HZ1, IT1 = (R1 + RP1 - 1) / RP1;
+    static_assert(RP2 >= 1 && RP1 >= 1, "a workgroup spans at least one halo row");
     float rt_reg[TAUMODE == 0 ? 1 : IT1];
     __shared__ double s_red[2][NT / 64];
 
@@ -82,6 +86,7 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     const int tid = threadIdx.x;
     const bool mom = tid >= NN;
     const int nid = mom ? tid - NN : tid;
+    __builtin_assume(nid >= 0 && nid < NN);
 
     const float* __restrict__ phi_in = a.mac_in;
     const float* __restrict__ p_in = a.mac_in + N;
@@ -91,41 +96,62 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
 
     // phase 0: this thread's 15 upstream populations
     const int tx = nid % TX, ty = (nid / TX) % TY, tz = nid / (TX * TY);
+    // 1-halo index of (x - 1, y - 1, z - 1): the upstream records of every
+    // direction lie at non-negative constant offsets from it
+    const int hb1 = tx + HX1 * (ty + HY1 * tz);
     const int x = x0 + tx, y = y0 + ty, z = z0 + tz;
     const bool inside = (x < nx) && (y < ny) && (z < nz);
     const int xc = inside ? x : 0, yc = inside ? y : 0, zc = inside ? z : 0;
     const int xs[3] = {xc + 1 == nx ? 0 : xc + 1, xc, xc == 0 ? nx - 1 : xc - 1};
     const int zs[3] = {zc + 1 == nz ? 0 : zc + 1, zc, zc == 0 ? nz - 1 : zc - 1};
-    const int64_t yo[3] = {(int64_t)nx * (yc + 1 == ny ? 0 : yc + 1), (int64_t)nx * yc,
-                           (int64_t)nx * (yc == 0 ? ny - 1 : yc - 1)};
-    const int64_t nxy = (int64_t)nx * ny;
-    const int64_t zo[3] = {nxy * zs[0], nxy * zs[1], nxy * zs[2]};
-    const float* __restrict__ pin = mom ? a.g_in : a.f_in;
+    // node indices as 32-bit factors (an x-y plane holds fewer than 2^31 nodes)
+    const uint32_t nxy32 = (uint32_t)nx * (uint32_t)ny;
+    const uint32_t yo[3] = {(uint32_t)(nx * (yc + 1 == ny ? 0 : yc + 1)), (uint32_t)(nx * yc),
+                            (uint32_t)(nx * (yc == 0 ? ny - 1 : yc - 1))};
+    const uint64_t zo[3] = {(uint64_t)(uint32_t)zs[0] * nxy32, (uint64_t)(uint32_t)zs[1] * nxy32,
+                            (uint64_t)(uint32_t)zs[2] * nxy32};
+    // node index of the upstream node x - c of direction q
+    auto upstream = [&](auto qc) -> int64_t {
+        using D = Dir<decltype(qc)::value>;
+        return (int64_t)(zo[1 + D::cz] + (uint64_t)(yo[1 + D::cy] + (uint32_t)xs[1 + D::cx]));
+    };
+    // the role's population array, as a scalar (the role is uniform per wavefront)
+    const float* __restrict__ pin = (__builtin_amdgcn_readfirstlane(mom ? 1 : 0) != 0) ? a.g_in : a.f_in;
     float ps[15];
     if (inside) {
         sfor32<0, 15>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            using D = Dir<q>;
-            ps[q] = pin[q * N + xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz]];
+            ps[q] = pin[q * N + upstream(qc)];
         });
     }
 
     // phase 1: stage fields (p and u into ga; p is replaced in phase 2)
+    // the rows of a tile whose halo box lies inside the grid in y and z need no wrap
     const bool wm = a.wrap_mod != 0;
-    for (int h = tid; h < H2; h += NT) {
-        const int hx = h % HX2, hy = (h / HX2) % HY2, hz = h / (HX2 * HY2);
-        const int gx = wrap32(x0 + hx - 2, nx, wm), gy = wrap32(y0 + hy - 2, ny, wm), gz = wrap32(z0 + hz - 2, nz, wm);
-        s_phi[h] = phi_in[gx + (int64_t)nx * (gy + (int64_t)ny * gz)];
+    const bool yz_in = !wm && y0 >= 2 && y0 + TY + 2 <= ny && z0 >= 2 && z0 + TZ + 2 <= nz;
+    auto row_offset = [&](int hy, int hz, int halo) -> int64_t {
+        int gy = y0 - halo + hy, gz = z0 - halo + hz;
+        if (!yz_in) {
+            gy = wrap32(gy, ny, wm);
+            gz = wrap32(gz, nz, wm);
+        }
+        return (int64_t)((uint64_t)(uint32_t)gz * nxy32 + (uint32_t)(gy * nx));
+    };
+    const int c2 = tid % HX2, r2 = tid / HX2, c1 = tid % HX1, r1 = tid / HX1;
+    const int gx2 = wrap32(x0 - 2 + c2, nx, wm), gx1 = wrap32(x0 - 1 + c1, nx, wm);
+    if (r2 < RP2) {
+        sfor32<0, IT2>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const int r = r2 + k * RP2;
+            if (r < R2) s_phi[c2 + HX2 * r] = phi_in[gx2 + row_offset(r % HY2, r / HY2, 2)];
+        });
     }
     sfor32<0, IT1>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int h = tid + k * NT;
-        if (h < H1) {
-            const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
-            const int gx = wrap32(x0 + hx - 1, nx, wm), gy = wrap32(y0 + hy - 1, ny, wm),
-                      gz = wrap32(z0 + hz - 1, nz, wm);
-            const int64_t g = gx + (int64_t)nx * (gy + (int64_t)ny * gz);
-            s_ga[h] = make_float4(p_in[g], u1_in[g], u2_in[g], u3_in[g]);
+        const int r = r1 + k * RP1;
+        if (r1 < RP1 && r < R1) {
+            const int64_t g = gx1 + row_offset(r % HY1, r / HY1, 1);
+            s_ga[c1 + HX1 * r] = make_float4(p_in[g], u1_in[g], u2_in[g], u3_in[g]);
             if constexpr (TAUMODE == 2) rt_reg[k] = 1.0f / a.tau_field[g];
         }
     });
@@ -138,16 +164,18 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     const float range = (float)(rmax - rmin), rminf = (float)(rmin - a.phi1);
     sfor32<0, IT1>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
-        const int h = tid + k * NT;
-        if (h >= H1) return;
-        const int hx = h % HX1, hy = (h / HX1) % HY1, hz = h / (HX1 * HY1);
-        const int c = (hx + 1) + HX2 * ((hy + 1) + HY2 * (hz + 1));
-        const float pt = s_phi[c];
+        const int r = r1 + k * RP1;
+        if (r1 >= RP1 || r >= R1) return;
+        const int h = c1 + HX1 * r;
+        // phi-box index of the node's (-1, -1, -1) neighbour: the stencil reads lie
+        // at non-negative constant offsets from it
+        const int cb = c1 + HX2 * (r % HY1 + HY2 * (r / HY1));
+        const float pt = s_phi[cb + (1 + HX2 * (1 + HY2))];
         double lap = -14.0 * (double)pt, g0 = 0.0, g1 = 0.0, g2 = 0.0;
         sfor32<1, 15>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             using D = Dir<q>;
-            const double v = (double)s_phi[c + D::cx + HX2 * (D::cy + HY2 * D::cz)];
+            const double v = (double)s_phi[cb + ((1 + D::cx) + HX2 * ((1 + D::cy) + HY2 * (1 + D::cz)))];
             lap += v;
             g0 = acc<D::cx>(g0, v);
             g1 = acc<D::cy>(g1, v);
@@ -185,7 +213,8 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     if constexpr (TAUMODE != 0) {
         sfor32<0, IT1>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
-            if (tid + k * NT < H1) s_rtau[tid + k * NT] = rt_reg[k];
+            const int r = r1 + k * RP1;
+            if (r1 < RP1 && r < R1) s_rtau[c1 + HX1 * r] = rt_reg[k];
         });
         __syncthreads();
     }
@@ -218,7 +247,7 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
                 constexpr float E = (float)weightE<q>(), Fw = (float)weightF<q>();
-                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
                 float fe;
                 if constexpr (q == 0) {
                     fe = __builtin_fmaf(Fw, s_fa[hself].x, TAUMODE != 0 ? phi_self : s_phi[h2self]);
@@ -241,7 +270,7 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
                 constexpr float E = (float)weightE<q>();
-                const int hu = (tx + 1 - D::cx) + HX1 * ((ty + 1 - D::cy) + HY1 * (tz + 1 - D::cz));
+                const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
                 float ge;
                 if constexpr (q == 0) {
                     ge = E * s_ga[hself].x;
@@ -260,7 +289,7 @@ __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
                         if constexpr (D::cx != 0) Fx = (float)(a.tg_amp * a.sinx[xu] * a.cosz[zu]);
                         if constexpr (D::cz != 0) Fz = (float)(-(a.tg_amp * a.cosx[xu] * a.sinz[zu]));
                     } else {
-                        const int64_t src = xs[1 + D::cx] + yo[1 + D::cy] + zo[1 + D::cz];
+                        const int64_t src = upstream(qc);
                         if constexpr (D::cx != 0) Fx = (float)a.fx[src];
                         if constexpr (D::cy != 0) Fy = (float)a.fy[src];
                         if constexpr (D::cz != 0) Fz = (float)a.fz[src];

@@ -87,9 +87,9 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     constexpr bool FASTDIV = FORCE != FORCE_FIELD && MODEL != MODEL_SCALE;
 
     __shared__ double s_phi[H2];
-    __shared__ double s_p[H1], s_u1[H1], s_u2[H1], s_u3[H1];
+    __shared__ double s_p3[H1], s_u1[H1], s_u2[H1], s_u3[H1];  // s_p3: 3 p
     __shared__ double s_A[H1], s_g0[H1], s_g1[H1], s_g2[H1];
-    // per-node 1.5 |grad phi|^2 (PRE_S) and |u|^2 (PRE_U), formed once with the
+    // per-node 1.5 |grad phi|^2 (PRE_S) and 1.5 |u|^2 (PRE_U), formed once with the
     // same operations the per-direction equilibria would repeat (bit-identical),
     // each only where it does not lower the number of workgroups that fit in a
     // CU's 160 KiB of LDS (sizes rounded up to a 1280-byte allocation granule)
@@ -101,7 +101,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     constexpr bool PRE_S = LDS_BASE + 8 * H1 <= 160 * 1024 && wg_per_cu_lds(LDS_BASE + 8 * H1) >= wg_per_cu_lds(LDS_BASE);
     constexpr bool PRE_U =
         PRE_S && LDS_BASE + 16 * H1 <= 160 * 1024 && wg_per_cu_lds(LDS_BASE + 16 * H1) >= wg_per_cu_lds(LDS_BASE);
-    __shared__ double s_s2h[PRE_S ? H1 : 1], s_usq[PRE_U ? H1 : 1];
+    __shared__ double s_s2h[PRE_S ? H1 : 1], s_usq15[PRE_U ? H1 : 1];  // 1.5 |grad phi|^2, 1.5 |u|^2
     __shared__ double s_tau[TAUMODE == TAU_CONST ? 1 : H1];
     __shared__ double s_red[2][NT / 64];
 
@@ -220,7 +220,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
         const int r = r1 + k * RP1;
         if (r1 < RP1 && r < R1) {
             const int h = c1 + HX1 * r;
-            s_p[h] = vp[k];
+            s_p3[h] = 3.0 * vp[k];
             s_u1[h] = vu1[k];
             s_u2[h] = vu2[k];
             s_u3[h] = vu3[k];
@@ -269,7 +269,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
         }
         s_g0[h] = g0;
         if constexpr (PRE_S) s_s2h[h] = 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
-        if constexpr (PRE_U) s_usq[h] = s_u1[h] * s_u1[h] + s_u2[h] * s_u2[h] + s_u3[h] * s_u3[h];
+        if constexpr (PRE_U) s_usq15[h] = 1.5 * (s_u1[h] * s_u1[h] + s_u2[h] * s_u2[h] + s_u3[h] * s_u3[h]);
         s_g1[h] = g1;
         s_g2[h] = g2;
         if constexpr (TAUMODE == TAU_PHI) {
@@ -423,21 +423,15 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 constexpr int q = decltype(qc)::value;
                 using D = Dir<q>;
                 const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
-                NodeEq<double> nd;
-                nd.p = s_p[hu];
-                nd.u1 = s_u1[hu];
-                nd.u2 = s_u2[hu];
-                nd.u3 = s_u3[hu];
-                const double usq = PRE_U ? s_usq[hu] : nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
+                const double u1 = s_u1[hu], u2 = s_u2[hu], u3 = s_u3[hu];
+                const double usq15 = PRE_U ? s_usq15[hu] : 1.5 * (u1 * u1 + u2 * u2 + u3 * u3);
                 double gt = 0.0;
                 if constexpr (q != 0) {
-                    nd.g0 = s_g0[hu];
-                    nd.g1 = s_g1[hu];
-                    nd.g2 = s_g2[hu];
-                    const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
-                    gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
+                    const double g0 = s_g0[hu], g1 = s_g1[hu], g2 = s_g2[hu];
+                    const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
+                    gt = korteweg<q>(g0, g1, g2, s2h);
                 }
-                const double ge = geq<q>(nd, usq, gt, a.ekg[wclass<q>()]);
+                const double ge = geq_pre<q>(s_p3[hu], u1, u2, u3, usq15, gt, a.ekg[wclass<q>()]);
                 const double dx = ps[q] - ge;
                 double gv;
                 if constexpr (TAUMODE != TAU_CONST) {
@@ -516,12 +510,8 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 // this step's g and the equilibria of its start at this node; Q = Pi:Pi
                 // (beta outer, alpha inner); sqrt(Q) is kept for the next step's tau
                 const int hs = (tx + 1) + HX1 * ((ty + 1) + HY1 * (tz + 1));
-                NodeEq<double> nd;
-                nd.p = s_p[hs];
-                nd.u1 = s_u1[hs];
-                nd.u2 = s_u2[hs];
-                nd.u3 = s_u3[hs];
-                const double usq = PRE_U ? s_usq[hs] : nd.u1 * nd.u1 + nd.u2 * nd.u2 + nd.u3 * nd.u3;
+                const double p3 = s_p3[hs], u1 = s_u1[hs], u2 = s_u2[hs], u3 = s_u3[hs];
+                const double usq15 = PRE_U ? s_usq15[hs] : 1.5 * (u1 * u1 + u2 * u2 + u3 * u3);
                 const double g0 = s_g0[hs], g1 = s_g1[hs], g2 = s_g2[hs];
                 const double s2h = PRE_S ? s_s2h[hs] : 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
                 double t[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
@@ -529,7 +519,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                     constexpr int q = decltype(qc)::value;
                     using D = Dir<q>;
                     constexpr int c[3] = {D::cx, D::cy, D::cz};
-                    const double d = ps[q] - geq<q>(nd, usq, korteweg<q>(g0, g1, g2, s2h), a.ekg[wclass<q>()]);
+                    const double d = ps[q] - geq_pre<q>(p3, u1, u2, u3, usq15, korteweg<q>(g0, g1, g2, s2h), a.ekg[wclass<q>()]);
 #pragma unroll
                     for (int b = 0; b < 3; b++)
 #pragma unroll
