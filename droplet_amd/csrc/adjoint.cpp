@@ -78,6 +78,7 @@ extern "C" int drp_tape_create(drp_lattice* L, int K, drp_tape** out) {
         return drp_fail(DRP_ERR_ARG, "the adjoint covers the free-energy model without scale-separated advection only");
     if (L->cfg.force_kind == DRP_FORCE_SPECTRAL)
         return drp_fail(DRP_ERR_ARG, "the adjoint does not cover the velocity-dependent spectral forcing");
+    if (L->n >= (int64_t)1 << 32) return drp_fail(DRP_ERR_ARG, "the adjoint supports grids of fewer than 2^32 nodes");
     const int wn = L->cfg.wall_axis == DRP_WALL_Y ? L->cfg.ny : L->cfg.nz;
     if (L->cfg.wall_axis != DRP_WALL_NONE && wn < 3)
         return drp_fail(DRP_ERR_ARG, "the adjoint needs at least 3 layers between the walls");
@@ -241,6 +242,11 @@ extern "C" int drp_tape_backward(drp_tape* T, const double w[2], double* loss, d
             a.T = L->phys.T;
             a.a = L->phys.a;
             a.b = L->phys.b;
+            a.rtau_f = 1.0 / a.tau_f;
+            a.rtau_g = 1.0 / a.tau_g;
+            a.rtau_f2 = 1.0 / (a.tau_f * a.tau_f);
+            a.om_f = 1.0 - 1.0 / a.tau_f;
+            a.om_g = 1.0 - 1.0 / a.tau_g;
             a.wall_axis = c.wall_axis;
             a.wall_active = (c.wall_axis != DRP_WALL_NONE && nstep >= c.wall_start) ? 1 : 0;
             a.first = (rel == 0 && T->start_eq) ? 1 : 0;

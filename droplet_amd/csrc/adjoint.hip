@@ -44,18 +44,21 @@ constexpr int ADJ_BLOCK = 256;
 
 __device__ __forceinline__ int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
+// node coordinates and indices in 32 bits: the tape admits fewer than 2^32 nodes
 struct Node {
     int x, y, z;
 };
-__device__ __forceinline__ Node coords(int64_t s, int nx, int ny) {
+__device__ __forceinline__ Node coords(uint32_t s, int nx, int ny) {
+    const uint32_t nxy = (uint32_t)nx * (uint32_t)ny;
     Node c;
-    c.x = (int)(s % nx);
-    c.y = (int)((s / nx) % ny);
-    c.z = (int)(s / ((int64_t)nx * ny));
+    c.z = (int)(s / nxy);
+    const uint32_t r = s - (uint32_t)c.z * nxy;
+    c.y = (int)(r / (uint32_t)nx);
+    c.x = (int)(r - (uint32_t)c.y * (uint32_t)nx);
     return c;
 }
-__device__ __forceinline__ int64_t shifted(const Node& c, int dx, int dy, int dz, int nx, int ny, int nz) {
-    return wrap1(c.x + dx, nx) + (int64_t)nx * (wrap1(c.y + dy, ny) + (int64_t)ny * wrap1(c.z + dz, nz));
+__device__ __forceinline__ uint32_t shifted(const Node& c, int dx, int dy, int dz, int nx, int ny, int nz) {
+    return (uint32_t)wrap1(c.x + dx, nx) + (uint32_t)nx * ((uint32_t)wrap1(c.y + dy, ny) + (uint32_t)ny * (uint32_t)wrap1(c.z + dz, nz));
 }
 
 // workgroup sum of three partials, added into acc[blockIdx.x][0..2] (each
@@ -81,9 +84,9 @@ __device__ void block_accumulate(double v0, double v1, double v2, double* acc) {
 __global__ __launch_bounds__(ADJ_BLOCK) void adj_collide_fp64(AdjArgs a) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t n = a.n;
-    const int64_t s = blockIdx.x * (int64_t)ADJ_BLOCK + threadIdx.x;
+    const uint32_t s = blockIdx.x * (uint32_t)ADJ_BLOCK + threadIdx.x;
     double d_tauf = 0.0, d_kg = 0.0, d_u = 0.0;
-    if (s < n) {
+    if (s < (uint64_t)n) {
         const Node c = coords(s, nx, ny);
         const double phi = a.mac[s];
         const double u[3] = {a.mac[2 * n + s], a.mac[3 * n + s], a.mac[4 * n + s]};
@@ -109,14 +112,16 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_collide_fp64(AdjArgs a) {
         const int wcoord_self = (a.wall_axis == 2) ? c.y : c.z;
         const int wn = (a.wall_axis == 2) ? ny : nz;
 
-        double phih = 0.0, Ah = 0.0, ph = 0.0, uh[3] = {0.0, 0.0, 0.0}, gh[3] = {0.0, 0.0, 0.0};
-        double Fp[15], Gp[15], gts[15];
-#pragma unroll
-        for (int q = 0; q < 15; q++) {
-            const int64_t d = shifted(c, CX[q], CY[q], CZ[q], nx, ny, nz);  // downstream node
-            double Fs = a.Fin[q * n + d];
+        // adjoints entering the post-collision populations of y, i.e. those of
+        // the streamed populations at the downstream node d = y + c_q, with the
+        // bounce-back transpose on wall layers.  Read twice (once for the node's
+        // adjoints, once for the outputs, the second time from L2) so that no
+        // per-direction array stays in registers.
+        auto adj_in = [&](int q, double& Fs, double& Gs) {
+            const uint32_t d = shifted(c, CX[q], CY[q], CZ[q], nx, ny, nz);
+            Fs = a.Fin[q * n + d];
             if (a.phin_in) Fs += a.phin_in[d];
-            double Gs = a.Gin[q * n + d];
+            Gs = a.Gin[q * n + d];
             if (a.wall_axis != 0) {
                 const int cw = (a.wall_axis == 2) ? wrap1(c.y + CY[q], ny) : wrap1(c.z + CZ[q], nz);
                 const bool bottom = cw == 0, top = cw == wn - 1;
@@ -127,16 +132,27 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_collide_fp64(AdjArgs a) {
                     else if (ca == -want) Gs += a.Gin[OPP[q] * n + d];          // copied into opp(q) at d
                 }
             }
+        };
+
+        double phih = 0.0, Ah = 0.0, ph = 0.0, uh[3] = {0.0, 0.0, 0.0}, gh[3] = {0.0, 0.0, 0.0}, d_kg0 = 0.0;
+        // rolled loops: a few loads in flight per lane and many lanes resident
+        // instead of all directions' loads held in registers
+#pragma unroll 1
+        for (int q = 0; q < 15; q++) {
+            double Fs, Gs;
+            adj_in(q, Fs, Gs);
             const double cu = CX[q] * u[0] + CY[q] * u[1] + CZ[q] * u[2];
             const double cg = CX[q] * gr[0] + CY[q] * gr[1] + CZ[q] * gr[2];
             const double c2 = (double)(CX[q] * CX[q] + CY[q] * CY[q] + CZ[q] * CZ[q]);
             const double gt = (q == 0) ? 0.0 : 4.5 * cg * cg - 1.5 * gsq * c2;
-            gts[q] = gt;
             const double fe = ((q == 0) ? phi : 0.0) + FW[q] * A + 3.0 * EW[q] * phi * cu + EW[q] * a.kf * gt;
-            const double fh = Fs / a.tau_f;   // adjoint of feq_q
-            const double geh = Gs / a.tau_g;  // adjoint of geq_q
-            d_tauf += Fs * (a.f[q * n + s] - fe) / (a.tau_f * a.tau_f);
+            const double fh = Fs * a.rtau_f;   // adjoint of feq_q
+            const double geh = Gs * a.rtau_g;  // adjoint of geq_q
+            d_tauf += Fs * (a.f[q * n + s] - fe) * a.rtau_f2;
             d_kg += geh * EW[q] * gt;
+            // populations of an initial state are geq / feq of constant fields:
+            // only the kappa_g dependence of g^0 = geq(m^0) remains
+            if (a.first) d_kg0 += a.om_g * Gs * EW[q] * gt;
             if (q == 0) phih += fh;
             Ah += FW[q] * fh;
             phih += fh * 3.0 * EW[q] * cu;
@@ -148,9 +164,8 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_collide_fp64(AdjArgs a) {
                 gh[k] += w * (9.0 * cg * cq[k] - 3.0 * c2 * gr[k]);
             }
             ph += 3.0 * EW[q] * geh;
-            Fp[q] = (1.0 - 1.0 / a.tau_f) * Fs;
-            Gp[q] = (1.0 - 1.0 / a.tau_g) * Gs;
         }
+        d_kg += d_kg0;
         // F-bracket A = p0(phi) - kf phi lap - kf/6 |grad|^2
         const double dp0 = a.T / (ombp * ombp) - 2.0 * a.a * phi;
         phih += Ah * (dp0 - a.kf * lap);
@@ -158,22 +173,23 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_collide_fp64(AdjArgs a) {
 #pragma unroll
         for (int k = 0; k < 3; k++) gh[k] += -2.0 * a.kf6 * gr[k] * Ah;
 
-        if (a.first) {
-            // populations of this state are geq / feq of constant initial fields:
-            // only the kappa_g dependence of g^0 = geq(m^0) remains
-#pragma unroll
-            for (int q = 0; q < 15; q++) {
-                d_kg += Gp[q] * EW[q] * gts[q];
-                a.Fout[q * n + s] = Fp[q];
-                a.Gout[q * n + s] = Gp[q];
+        const double p3 = ph / 3.0;
+        // the second read is a real one (not the first pass's values kept in registers)
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+#pragma unroll 1
+        for (int q = 0; q < 15; q++) {
+            double Fs, Gs;
+            adj_in(q, Fs, Gs);
+            const double Fp = a.om_f * Fs, Gp = a.om_g * Gs;
+            if (a.first) {
+                a.Fout[q * n + s] = Fp;
+                a.Gout[q * n + s] = Gp;
+            } else {
+                a.Fout[q * n + s] = Fp + phih;
+                a.Gout[q * n + s] = Gp + p3 + (CX[q] * uh[0] + CY[q] * uh[1] + CZ[q] * uh[2]);
             }
-        } else {
-            const double p3 = ph / 3.0;
-#pragma unroll
-            for (int q = 0; q < 15; q++) {
-                a.Fout[q * n + s] = Fp[q] + phih;
-                a.Gout[q * n + s] = Gp[q] + p3 + (CX[q] * uh[0] + CY[q] * uh[1] + CZ[q] * uh[2]);
-            }
+        }
+        if (!a.first) {
             a.lapA[s] = laph;
             a.gradA[s] = gh[0];
             a.gradA[n + s] = gh[1];
@@ -205,8 +221,8 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_stencil_fp64(int nx, int ny, in
                                                               const double* __restrict__ gradA,
                                                               double* __restrict__ out) {
     const int64_t n = (int64_t)nx * ny * nz;
-    const int64_t s = blockIdx.x * (int64_t)ADJ_BLOCK + threadIdx.x;
-    if (s >= n) return;
+    const uint32_t s = blockIdx.x * (uint32_t)ADJ_BLOCK + threadIdx.x;
+    if (s >= (uint64_t)n) return;
     const Node c = coords(s, nx, ny);
     double l = 0.0, g = 0.0;
 #pragma unroll
@@ -223,8 +239,8 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adj_seed_area_fp64(int nx, int ny, 
                                                                 double w, double* __restrict__ lapA,
                                                                 double* __restrict__ gradA) {
     const int64_t n = (int64_t)nx * ny * nz;
-    const int64_t s = blockIdx.x * (int64_t)ADJ_BLOCK + threadIdx.x;
-    if (s >= n) return;
+    const uint32_t s = blockIdx.x * (uint32_t)ADJ_BLOCK + threadIdx.x;
+    if (s >= (uint64_t)n) return;
     const Node c = coords(s, nx, ny);
     double gr[3] = {0.0, 0.0, 0.0};
 #pragma unroll

@@ -184,6 +184,8 @@ struct AdjArgs {
     double* gradA;          // adjoint of grad phi at step n (3n)
     double* acc;            // [workgroups][3]: d/dtau_f, d/dkappa_g, d/dU_wall, accumulated
     double tau_f, tau_g, kf, kf6, kg, T, a, b;
+    double rtau_f, rtau_g, rtau_f2;  // 1 / tau_f, 1 / tau_g, 1 / (tau_f tau_f)
+    double om_f, om_g;               // 1 - 1 / tau_f, 1 - 1 / tau_g
     int wall_axis;          // 0, 2 (y) or 3 (z)
     int wall_active;        // the walls move during step n -> n+1
     int first;              // step n is the initial equilibrium of constant fields
