@@ -82,9 +82,10 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
     constexpr int HX1 = TX + 2, HY1 = TY + 2, HZ1 = TZ + 2, H1 = HX1 * HY1 * HZ1;
     static_assert(NN % 64 == 0, "a role must own whole wavefronts");
     // relaxation by the shared-divisor division (d3q15.h) where it costs no
-    // spilled registers: not with the register rings of a force field or of the
-    // band-passed velocity
-    constexpr bool FASTDIV = FORCE != FORCE_FIELD && MODEL != MODEL_SCALE;
+    // spilled registers: not in a role that carries a register ring (the
+    // band-passed velocity of the phi role, the force field of the momentum role)
+    constexpr bool FASTDIV_F = MODEL != MODEL_SCALE;
+    constexpr bool FASTDIV_G = FORCE != FORCE_FIELD;
 
     __shared__ double s_phi[H2];
     __shared__ double s_p3[H1], s_u1[H1], s_u2[H1], s_u3[H1];  // s_p3: 3 p
@@ -388,7 +389,7 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
             });
             };
-            if constexpr (FASTDIV) {
+            if constexpr (FASTDIV_F) {
                 collide(std::false_type{});
                 if (redo) collide(std::true_type{});
             } else {
@@ -465,8 +466,8 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
             });
             };
-            collide(std::integral_constant<bool, !FASTDIV>{});
-            if constexpr (FASTDIV && TAUMODE == TAU_CONST) {
+            collide(std::integral_constant<bool, !FASTDIV_G>{});
+            if constexpr (FASTDIV_G && TAUMODE == TAU_CONST) {
                 if (redo) {
                     // the populations this step started from, again, addressed from
                     // the node's coordinates (so that phase 1's upstream offsets
