@@ -90,7 +90,8 @@ extern "C" {
 #define DRP_POP_MOM 1 /* reference g */
 
 typedef struct drp_config {
-    int32_t nx, ny, nz;   /* lattice nodes per axis (reference xmax+1, ymax+1, zmax+1) */
+    int32_t nx, ny, nz;   /* lattice nodes per axis (reference xmax+1, ymax+1, zmax+1); each >= 3,
+                             nx * ny < 2^31 */
     int32_t precision;    /* DRP_FP64 | DRP_FP32_SHIFTED                              */
     double tau_f;         /* phi-population relaxation time (ref tauf)                 */
     double nu1, nu2;      /* kinematic viscosity of continuous / dispersed phase       */
@@ -291,7 +292,7 @@ void drp_batch_destroy(drp_batch* b);
  * Gradient entries (DRP_GRAD_*): the phi-population mobility M = tau_f - 1/2
  * (Inamuro model, lattice units: dJ/dM = dJ/dtau_f), kappa_g (surface tension,
  * sigma = 1.7039 kappa_g^0.9991) and the wall speed U (shear rate 2U/H).
- * Requirements: FP64, nu1 == nu2, no LES.  When the tape starts right after
+ * Requirements: FP64, nu1 == nu2, no LES, fewer than 2^32 nodes.  When the tape starts right after
  * drp_init_droplets / drp_init_fields, the kappa_g dependence of the initial
  * momentum equilibria is included.  The lattice must be stepped only through
  * the tape while it is recording. */
