@@ -69,6 +69,7 @@ def test_invalid_config_is_rejected_without_gpu_work():
     (dict(advect_k_high=6), b"advect_k_high"),
     (dict(advect_k_high=2, advect_k_low=2), b"advect_k_low"),
     (dict(tau_f=0.5), b"tau_f"),
+    (dict(nx=65536, ny=32768, nz=3), b"x-y plane"),
 ])
 def test_config_fields_are_validated_without_gpu_work(kw, msg):
     lib = C.CDLL(LIB)
