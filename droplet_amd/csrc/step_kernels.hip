@@ -344,50 +344,50 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
                 if constexpr (D::cz != 0) u[2] = a.uadv[2 * N + src];
             };
             auto collide = [&](auto full_c) {
-            constexpr bool FULL = decltype(full_c)::value;
-            if constexpr (MODEL == MODEL_SCALE)
-                static_for<1, 1 + UPF>([&](auto qc) { load_uadv(qc); });
-            static_for<0, 15>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                using D = Dir<q>;
-                const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
-                const int h2 = hb2 + ((2 - D::cx) + HX2 * ((2 - D::cy) + HY2 * (2 - D::cz)));
-                NodeEq<double> nd;
-                nd.phi = s_phi[h2];
-                nd.A = s_A[hu];
-                double gt = 0.0;
-                if constexpr (q != 0) {
-                    if constexpr (MODEL == MODEL_SCALE) {
-                        // only the components with c_a != 0 enter c . u
-                        const double* u = uring[q % UPF];
-                        nd.u1 = u[0];
-                        nd.u2 = u[1];
-                        nd.u3 = u[2];
-                        if constexpr (q + UPF < 15) load_uadv(std::integral_constant<int, q + UPF>{});
-                    } else {
-                        nd.u1 = s_u1[hu];
-                        nd.u2 = s_u2[hu];
-                        nd.u3 = s_u3[hu];
+                constexpr bool FULL = decltype(full_c)::value;
+                if constexpr (MODEL == MODEL_SCALE)
+                    static_for<1, 1 + UPF>([&](auto qc) { load_uadv(qc); });
+                static_for<0, 15>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    using D = Dir<q>;
+                    const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
+                    const int h2 = hb2 + ((2 - D::cx) + HX2 * ((2 - D::cy) + HY2 * (2 - D::cz)));
+                    NodeEq<double> nd;
+                    nd.phi = s_phi[h2];
+                    nd.A = s_A[hu];
+                    double gt = 0.0;
+                    if constexpr (q != 0) {
+                        if constexpr (MODEL == MODEL_SCALE) {
+                            // only the components with c_a != 0 enter c . u
+                            const double* u = uring[q % UPF];
+                            nd.u1 = u[0];
+                            nd.u2 = u[1];
+                            nd.u3 = u[2];
+                            if constexpr (q + UPF < 15) load_uadv(std::integral_constant<int, q + UPF>{});
+                        } else {
+                            nd.u1 = s_u1[hu];
+                            nd.u2 = s_u2[hu];
+                            nd.u3 = s_u3[hu];
+                        }
+                        nd.g0 = s_g0[hu];
+                        nd.g1 = s_g1[hu];
+                        nd.g2 = s_g2[hu];
+                        const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
+                        gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
                     }
-                    nd.g0 = s_g0[hu];
-                    nd.g1 = s_g1[hu];
-                    nd.g2 = s_g2[hu];
-                    const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (nd.g0 * nd.g0 + nd.g1 * nd.g1 + nd.g2 * nd.g2);
-                    gt = korteweg<q>(nd.g0, nd.g1, nd.g2, s2h);
-                }
-                const double fe = feq<q>(nd, gt, a.ekf[wclass<q>()]);
-                const double dx = ps[q] - fe;
-                double fv;
-                if constexpr (FULL) {
-                    fv = ps[q] - dx / a.tau_f;
-                } else {
-                    fv = ps[q] - div_shared_fast(dx, dtau);
-                    redo |= !div_shared_exact(dx);
-                }
-                ph = (q == 0) ? fv : ph + fv;
-                __builtin_nontemporal_store(fv, &a.f_out[q * N + s]);
-                __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
-            });
+                    const double fe = feq<q>(nd, gt, a.ekf[wclass<q>()]);
+                    const double dx = ps[q] - fe;
+                    double fv;
+                    if constexpr (FULL) {
+                        fv = ps[q] - dx / a.tau_f;
+                    } else {
+                        fv = ps[q] - div_shared_fast(dx, dtau);
+                        redo |= !div_shared_exact(dx);
+                    }
+                    ph = (q == 0) ? fv : ph + fv;
+                    __builtin_nontemporal_store(fv, &a.f_out[q * N + s]);
+                    __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
+                });
             };
             if constexpr (FASTDIV_F) {
                 collide(std::false_type{});
@@ -418,53 +418,53 @@ __device__ __forceinline__ void step_tile_fp64(const StepArgs& a, int tile) {
             const SharedDivisor dtau = shared_divisor(a.tau_const);
             bool redo = TAUMODE == TAU_CONST && !dtau.ok;
             auto collide = [&](auto full_c) {
-            constexpr bool FULL = decltype(full_c)::value;
-            if constexpr (FORCE == FORCE_FIELD) static_for<1, 1 + FPF>(load_force);
-            static_for<0, 15>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                using D = Dir<q>;
-                const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
-                const double u1 = s_u1[hu], u2 = s_u2[hu], u3 = s_u3[hu];
-                const double usq15 = PRE_U ? s_usq15[hu] : 1.5 * (u1 * u1 + u2 * u2 + u3 * u3);
-                double gt = 0.0;
-                if constexpr (q != 0) {
-                    const double g0 = s_g0[hu], g1 = s_g1[hu], g2 = s_g2[hu];
-                    const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
-                    gt = korteweg<q>(g0, g1, g2, s2h);
-                }
-                const double ge = geq_pre<q>(s_p3[hu], u1, u2, u3, usq15, gt, a.ekg[wclass<q>()]);
-                const double dx = ps[q] - ge;
-                double gv;
-                if constexpr (TAUMODE != TAU_CONST) {
-                    gv = ps[q] - dx / s_tau[hu];
-                } else if constexpr (FULL) {
-                    gv = ps[q] - dx / a.tau_const;
-                } else {
-                    gv = ps[q] - div_shared_fast(dx, dtau);
-                    redo |= !div_shared_exact(dx);
-                }
-                if constexpr (FORCE != FORCE_NONE && q != 0) {
-                    // c . F with the zero-velocity terms dropped, as for c . u: the
-                    // reference's (cx Fx + cy Fy) + cz Fz rounds identically, and
-                    // only the components that enter are loaded
-                    double Fx = 0.0, Fy = 0.0, Fz = 0.0;
-                    if constexpr (FORCE == FORCE_TAYLOR_GREEN) {
-                        const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
-                        if constexpr (D::cx != 0) Fx = a.tg_amp * a.sinx[xu] * a.cosz[zu];
-                        if constexpr (D::cz != 0) Fz = -(a.tg_amp * a.cosx[xu] * a.sinz[zu]);
-                    } else {  // field at the upstream node, loaded FPF directions ahead
-                        const double* f = fring[q % FPF];
-                        if constexpr (D::cx != 0) Fx = f[0];
-                        if constexpr (D::cy != 0) Fy = f[1];
-                        if constexpr (D::cz != 0) Fz = f[2];
-                        if constexpr (q + FPF < 15) load_force(std::integral_constant<int, q + FPF>{});
+                constexpr bool FULL = decltype(full_c)::value;
+                if constexpr (FORCE == FORCE_FIELD) static_for<1, 1 + FPF>(load_force);
+                static_for<0, 15>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    using D = Dir<q>;
+                    const int hu = hb1 + ((1 - D::cx) + HX1 * ((1 - D::cy) + HY1 * (1 - D::cz)));
+                    const double u1 = s_u1[hu], u2 = s_u2[hu], u3 = s_u3[hu];
+                    const double usq15 = PRE_U ? s_usq15[hu] : 1.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+                    double gt = 0.0;
+                    if constexpr (q != 0) {
+                        const double g0 = s_g0[hu], g1 = s_g1[hu], g2 = s_g2[hu];
+                        const double s2h = PRE_S ? s_s2h[hu] : 1.5 * (g0 * g0 + g1 * g1 + g2 * g2);
+                        gt = korteweg<q>(g0, g1, g2, s2h);
                     }
-                    const double dot = cdot<q>(Fx, Fy, Fz);
-                    gv = gv + 3.0 * weightE<q>() * dot;
-                }
-                ps[q] = gv;
-                __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
-            });
+                    const double ge = geq_pre<q>(s_p3[hu], u1, u2, u3, usq15, gt, a.ekg[wclass<q>()]);
+                    const double dx = ps[q] - ge;
+                    double gv;
+                    if constexpr (TAUMODE != TAU_CONST) {
+                        gv = ps[q] - dx / s_tau[hu];
+                    } else if constexpr (FULL) {
+                        gv = ps[q] - dx / a.tau_const;
+                    } else {
+                        gv = ps[q] - div_shared_fast(dx, dtau);
+                        redo |= !div_shared_exact(dx);
+                    }
+                    if constexpr (FORCE != FORCE_NONE && q != 0) {
+                        // c . F with the zero-velocity terms dropped, as for c . u: the
+                        // reference's (cx Fx + cy Fy) + cz Fz rounds identically, and
+                        // only the components that enter are loaded
+                        double Fx = 0.0, Fy = 0.0, Fz = 0.0;
+                        if constexpr (FORCE == FORCE_TAYLOR_GREEN) {
+                            const int xu = xs[1 + D::cx], zu = zs[1 + D::cz];
+                            if constexpr (D::cx != 0) Fx = a.tg_amp * a.sinx[xu] * a.cosz[zu];
+                            if constexpr (D::cz != 0) Fz = -(a.tg_amp * a.cosx[xu] * a.sinz[zu]);
+                        } else {  // field at the upstream node, loaded FPF directions ahead
+                            const double* f = fring[q % FPF];
+                            if constexpr (D::cx != 0) Fx = f[0];
+                            if constexpr (D::cy != 0) Fy = f[1];
+                            if constexpr (D::cz != 0) Fz = f[2];
+                            if constexpr (q + FPF < 15) load_force(std::integral_constant<int, q + FPF>{});
+                        }
+                        const double dot = cdot<q>(Fx, Fy, Fz);
+                        gv = gv + 3.0 * weightE<q>() * dot;
+                    }
+                    ps[q] = gv;
+                    __builtin_amdgcn_sched_barrier(0);  // keep each direction's LDS reads local
+                });
             };
             collide(std::integral_constant<bool, !FASTDIV_G>{});
             if constexpr (FASTDIV_G && TAUMODE == TAU_CONST) {
