@@ -81,7 +81,8 @@ __device__ void block_accumulate(double v0, double v1, double v2, double* acc) {
 }
 }  // namespace
 
-__global__ __launch_bounds__(ADJ_BLOCK) void adj_collide_fp64(AdjArgs a) {
+// five wavefronts per SIMD: 96 VGPRs hold the rolled direction loops without spilling
+__global__ __launch_bounds__(ADJ_BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void adj_collide_fp64(AdjArgs a) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t n = a.n;
     const uint32_t s = blockIdx.x * (uint32_t)ADJ_BLOCK + threadIdx.x;
