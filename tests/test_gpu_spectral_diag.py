@@ -81,8 +81,11 @@ def _random_state(shape, seed, smooth=False):
     return phi, np.full(shape, 1 / 3), u1, u2, u3
 
 
-@pytest.mark.parametrize("shape", [(16, 16, 16), (30, 20, 24), (12, 36, 40), (64, 64, 64)])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (30, 20, 24), (12, 36, 40), (64, 64, 64),
+                                   (4, 6, 2048), (3, 1500, 4), (1024, 5, 3)])
 def test_fft3d_matches_numpy(shape):
+    """Including lines longer than one 1024-point LDS block (2048 along x, 1500
+    along y) and of exactly one block (1024 along z), one line per workgroup."""
     from droplet_amd.lattice import fft3d
     rng = np.random.default_rng(sum(shape))
     a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
