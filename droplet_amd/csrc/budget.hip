@@ -9,12 +9,12 @@
 //                            step started from (the other ping-pong buffer) and the
 //                            fields it produced
 //
-// Neither is on the per-step hot path.  The budget runs in two passes over the
-// nodes: the first forms the per-node terms that need no stencil of a derived
-// quantity (left1, right3, right5) and stores K = |u|^2/2, u_X S_XY and the
-// Korteweg tensor G_XY (19 doubles per node, allocated on the first call); the
-// second takes the D3Q15 gradients of those and of p (left2, right1, right2,
-// right4).  Each workgroup leaves its partial sums; the host adds them with
+// Neither is on the step's hot path (energy_budget.f90 evaluates the budget after
+// every step).  The budget runs in two passes over the nodes: the first forms the per-node terms that need no stencil of a derived
+// quantity (left1, right3, right5) and stores K = |u|^2/2, V_Y = u_X S_XY and
+// grad phi (7 doubles per node, allocated on the first call); the second takes
+// the D3Q15 gradients of those, of p and of the Korteweg tensor G_XY formed at
+// each stencil node (left2, right1, right2, right4).  Each workgroup leaves its partial sums; the host adds them with
 // Neumaier compensation in workgroup order.  Every per-node expression keeps
 // the reference's evaluation order (FMA contraction is off for this file).
 #include <hip/hip_runtime.h>
@@ -52,23 +52,22 @@ struct BGrid {
         z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
         return x + (int64_t)nx * (y + (int64_t)ny * z);
     }
+    // byte offset of a double at (x, y, z), wrapped; valid for grids of fewer than
+    // 2^29 nodes, where it fits 32 bits (loads then take a uniform base address
+    // and a 32-bit lane offset)
+    __device__ __forceinline__ uint32_t off8(int x, int y, int z) const {
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
+        return 8u * ((uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z));
+    }
 };
-
-// grad_cal along axis A: sum over the directions of c_A v(x + c), over 10
-template <int A> __device__ __forceinline__ double grad_along(const BGrid& G, const double* v, int x, int y, int z) {
-    double s = 0.0;
-    sfor<1, 15>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        using D = Dir<q>;
-        constexpr int c = A == 0 ? D::cx : (A == 1 ? D::cy : D::cz);
-        if constexpr (c != 0) s = acc<c>(s, v[G.at(x + D::cx, y + D::cy, z + D::cz)]);
-    });
-    return s / 10.0;
+__device__ __forceinline__ double ld8(const double* base, uint32_t off) {
+    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
 }
 
-__device__ __forceinline__ double grad_axis(const BGrid& G, const double* v, int x, int y, int z, int a) {
-    return a == 0 ? grad_along<0>(G, v, x, y, z) : (a == 1 ? grad_along<1>(G, v, x, y, z) : grad_along<2>(G, v, x, y, z));
-}
+// component A of direction Q
+template <int Q, int A> constexpr int cdir() { return A == 0 ? Dir<Q>::cx : (A == 1 ? Dir<Q>::cy : Dir<Q>::cz); }
 
 // lap_cal and grad_cal of phi, grad of u (gu[k][a] = d_a u_k) at one node
 __device__ void stencils(const BGrid& G, const double* phi, const double* u1, const double* u2, const double* u3,
@@ -123,8 +122,15 @@ __global__ __launch_bounds__(256) void derived_fp64(BGrid G, Phys P, const doubl
     }
 }
 
-constexpr int BPART = 8;   // partial sums per workgroup: the seven terms and a pad
-constexpr int BWORK = 19;  // doubles per node: K, u_X S_XY (9), G_XY (9)
+constexpr int BPART = 8;  // partial sums per workgroup: the seven terms and a pad
+// Work space, doubles per node: K = |u|^2/2 of the new u, V_Y = sum_X u_X S_XY
+// (new u, strain of the old fields) and grad phi of the old fields.  right2 is
+// taken as sum_Y d_Y V_Y, the reference's sum_Y sum_X d_Y (u_X S_XY) with the
+// sum over X moved inside the (linear) stencil: equal but for rounding, and 3
+// arrays instead of 9.  G_XY is formed at each stencil node from grad phi there
+// with the expression pass 1 would use.
+constexpr int BWORK = 7;
+enum { W_K = 0, W_V = 1, W_GP = 4 };
 
 struct BudgetArgs {
     BGrid G;
@@ -164,14 +170,43 @@ template <int K0, int K1> __device__ __forceinline__ void block_partials(double 
     }
 }
 
+// Workgroup brick of the budget passes: 64 x 4 columns (a wavefront per x row)
+// that march over BZC planes, so the z neighbours of a plane's nodes were read by
+// the same workgroup one or two iterations earlier and are still in L2; bricks are
+// dealt to the XCDs in contiguous runs (xcd_tile), so bricks adjacent in y share
+// their halo rows in one L2.
+constexpr int BTX = 64, BTY = 4, BZC = 16;
+
+__host__ __device__ inline int64_t budget_bricks(int nx, int ny, int nz) {
+    return (int64_t)((nx + BTX - 1) / BTX) * ((ny + BTY - 1) / BTY) * ((nz + BZC - 1) / BZC);
+}
+
+struct Brick {
+    int x, y, z0, z1;
+    bool live;  // x < nx and y < ny
+};
+__device__ __forceinline__ Brick brick(const BGrid& G) {
+    const int bx = (G.nx + BTX - 1) / BTX, by = (G.ny + BTY - 1) / BTY;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int tx = t % bx, ty = (t / bx) % by, tz = t / (bx * by);
+    Brick b;
+    b.x = tx * BTX + (int)(threadIdx.x % BTX);
+    b.y = ty * BTY + (int)(threadIdx.x / BTX);
+    b.z0 = tz * BZC;
+    b.z1 = min(b.z0 + BZC, G.nz);
+    b.live = b.x < G.nx && b.y < G.ny;
+    return b;
+}
+
 __global__ __launch_bounds__(256) void budget_pass1(BudgetArgs a) {
     const BGrid& G = a.G;
     const Phys& P = a.P;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const Brick b = brick(G);
     double v[BPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (s < n) {
-        const int x = (int)(s % G.nx), y = (int)((s / G.nx) % G.ny), z = (int)(s / ((int64_t)G.nx * G.ny));
+    for (int z = b.z0; b.live && z < b.z1; z++) {
+        const int x = b.x, y = b.y;
+        const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
         const double* om = a.old_mac;
         const double* nm = a.new_mac;
         double lap, gp[3], gu[3][3];
@@ -183,15 +218,15 @@ __global__ __launch_bounds__(256) void budget_pass1(BudgetArgs a) {
         const double un[3] = {nm[2 * n + s], nm[3 * n + s], nm[4 * n + s]};
         // left1 (kinetic_former with phi_procs already the new phi)
         const double former = p0 - P.kg * phn * lap - 1.0 / 6.0 * P.kg * gsq;
-        v[0] = fabs(nm[n + s] - former);
+        v[0] += fabs(nm[n + s] - former);
         double S[3][3];
-        for (int b = 0; b < 3; b++)
-            for (int c = 0; c < 3; c++) S[c][b] = 0.5 * (gu[c][b] + gu[b][c]);
+        for (int bb = 0; bb < 3; bb++)
+            for (int c = 0; c < 3; c++) S[c][bb] = 0.5 * (gu[c][bb] + gu[bb][c]);
         const double nu = nu_new(a, phn);
         double r3 = 0.0;
-        for (int b = 0; b < 3; b++)
-            for (int c = 0; c < 3; c++) r3 = r3 + S[c][b] * S[c][b];
-        v[4] = r3 * (-2.0) * nu;
+        for (int bb = 0; bb < 3; bb++)
+            for (int c = 0; c < 3; c++) r3 = r3 + S[c][bb] * S[c][bb];
+        v[4] += r3 * (-2.0) * nu;
         double F[3] = {0.0, 0.0, 0.0};
         if (a.fx) {
             F[0] = a.fx[s];
@@ -201,14 +236,11 @@ __global__ __launch_bounds__(256) void budget_pass1(BudgetArgs a) {
             F[0] = a.tg_amp * a.sinx[x] * a.cosz[z];
             F[2] = -(a.tg_amp * a.cosx[x] * a.sinz[z]);
         }
-        v[6] = un[0] * F[0] + un[1] * F[1] + un[2] * F[2];
+        v[6] += un[0] * F[0] + un[1] * F[1] + un[2] * F[2];
         double* w = a.work;
-        w[s] = 0.5 * (un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
-        for (int X = 0; X < 3; X++)
-            for (int Y = 0; Y < 3; Y++) {
-                w[(1 + X * 3 + Y) * n + s] = un[X] * S[X][Y];
-                w[(10 + X * 3 + Y) * n + s] = 4.5 * gp[X] * gp[Y] - 1.5 * gsq * (X == Y ? 1.0 : 0.0);
-            }
+        w[W_K * n + s] = 0.5 * (un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
+        for (int Y = 0; Y < 3; Y++) w[(W_V + Y) * n + s] = un[0] * S[0][Y] + un[1] * S[1][Y] + un[2] * S[2][Y];
+        for (int X = 0; X < 3; X++) w[(W_GP + X) * n + s] = gp[X];
     }
     block_partials<0, BPART>(v, a.partial);
 }
@@ -217,27 +249,63 @@ __global__ __launch_bounds__(256) void budget_pass2(BudgetArgs a) {
     const BGrid& G = a.G;
     const Phys& P = a.P;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
-    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const Brick b = brick(G);
     double v[BPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (s < n) {
-        const int x = (int)(s % G.nx), y = (int)((s / G.nx) % G.ny), z = (int)(s / ((int64_t)G.nx * G.ny));
+    for (int z = b.z0; b.live && z < b.z1; z++) {
+        const int x = b.x, y = b.y;
+        const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
         const double* nm = a.new_mac;
         const double* w = a.work;
-        const double u[3] = {nm[2 * n + s], nm[3 * n + s], nm[4 * n + s]};
-        v[1] = u[0] * grad_along<0>(G, w, x, y, z) + u[1] * grad_along<1>(G, w, x, y, z) +
-               u[2] * grad_along<2>(G, w, x, y, z);
         const double* p = nm + n;
-        const double r1 = u[0] * grad_along<0>(G, p, x, y, z) + u[1] * grad_along<1>(G, p, x, y, z) +
-                          u[2] * grad_along<2>(G, p, x, y, z);
-        v[2] = -1.0 * r1;
-        double r2 = 0.0, r4 = 0.0;
-        for (int Y = 0; Y < 3; Y++)
-            for (int X = 0; X < 3; X++) {
-                r2 = r2 + grad_axis(G, w + (1 + X * 3 + Y) * n, x, y, z, Y);
-                r4 = r4 + u[X] * grad_axis(G, w + (10 + X * 3 + Y) * n, x, y, z, Y);
+        const double* wK = w + W_K * n;
+        const double* wV[3] = {w + W_V * n, w + (W_V + 1) * n, w + (W_V + 2) * n};
+        const double* wg[3] = {w + W_GP * n, w + (W_GP + 1) * n, w + (W_GP + 2) * n};
+        // grad_cal sums (sum over the directions of c_A v(x + c), in direction
+        // order) of K, p and G_XA along every axis A and of V_A along A, in one
+        // pass over the 14 neighbours; G_XA is formed at the neighbour
+        double aK[3] = {0.0, 0.0, 0.0}, aP[3] = {0.0, 0.0, 0.0}, aV[3] = {0.0, 0.0, 0.0};
+        double aG[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+        sfor<1, 15>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            using D = Dir<q>;
+            const uint32_t t = G.off8(x + D::cx, y + D::cy, z + D::cz);
+            const double Kt = ld8(wK, t), pt = ld8(p, t);
+            const double gt[3] = {ld8(wg[0], t), ld8(wg[1], t), ld8(wg[2], t)};
+            const double gsq = gt[0] * gt[0] + gt[1] * gt[1] + gt[2] * gt[2];
+            sfor<0, 3>([&](auto Ac) {
+                constexpr int A = decltype(Ac)::value;
+                constexpr int c = cdir<q, A>();
+                if constexpr (c != 0) {
+                    aK[A] = acc<c>(aK[A], Kt);
+                    aP[A] = acc<c>(aP[A], pt);
+                    aV[A] = acc<c>(aV[A], ld8(wV[A], t));
+                    sfor<0, 3>([&](auto Xc) {
+                        constexpr int X = decltype(Xc)::value;
+                        aG[X][A] = acc<c>(aG[X][A], 4.5 * gt[X] * gt[A] - 1.5 * gsq * (X == A ? 1.0 : 0.0));
+                    });
+                }
+            });
+            // Pin the sums after each neighbour, and fence memory after every
+            // second one: left alone, the compiler issues all 14 neighbours'
+            // loads first and holds them (over 300 VGPRs, one wave per SIMD).
+#pragma unroll
+            for (int A = 0; A < 3; A++) {
+                asm volatile("" : "+v"(aK[A]), "+v"(aP[A]), "+v"(aV[A]));
+                asm volatile("" : "+v"(aG[0][A]), "+v"(aG[1][A]), "+v"(aG[2][A]));
             }
-        v[3] = r2 * 2.0 * nu_new(a, nm[s]);
-        v[5] = r4 * (-1.0) * (2.0 / 9.0) * P.kg;
+            if constexpr (q % 2 == 0) asm volatile("" ::: "memory");
+        });
+        const double u[3] = {nm[2 * n + s], nm[3 * n + s], nm[4 * n + s]};
+        v[1] += u[0] * (aK[0] / 10.0) + u[1] * (aK[1] / 10.0) + u[2] * (aK[2] / 10.0);
+        const double r1 = u[0] * (aP[0] / 10.0) + u[1] * (aP[1] / 10.0) + u[2] * (aP[2] / 10.0);
+        v[2] += -1.0 * r1;
+        double r2 = 0.0, r4 = 0.0;
+        for (int Y = 0; Y < 3; Y++) {
+            r2 = r2 + aV[Y] / 10.0;
+            for (int X = 0; X < 3; X++) r4 = r4 + u[X] * (aG[X][Y] / 10.0);
+        }
+        v[3] += r2 * 2.0 * nu_new(a, nm[s]);
+        v[5] += r4 * (-1.0) * (2.0 / 9.0) * P.kg;
     }
     // pass 1 wrote columns 0, 4, 6 (and zeros in the others); add this pass's columns
     __shared__ double sh[BPART][4];
@@ -301,7 +369,8 @@ extern "C" int drp_step_energy_budget(drp_lattice* L, double* terms) {
     if (c.precision != DRP_FP64) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: FP64 lattices only");
     if (c.model != DRP_MODEL_FREE_ENERGY) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: free-energy model only");
     const int64_t n = L->n;
-    const int64_t nb = (n + 255) / 256;
+    if (n >= ((int64_t)1 << 29)) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: grids of 2^29 or more nodes");
+    const int64_t nb = budget_bricks(c.nx, c.ny, c.nz);
     if (!L->budget) {
         HIP_TRYB(hipMalloc((void**)&L->budget, ((size_t)BWORK * n + (size_t)nb * BPART) * sizeof(double)));
     }

@@ -7,6 +7,14 @@
 namespace drp {
 
 enum { FORCE_NONE = 0, FORCE_TAYLOR_GREEN = 1, FORCE_FIELD = 2 };
+
+// Map the linear workgroup id to a tile so that each XCD (workgroups are dealt
+// round-robin to the 8 XCDs) gets a contiguous block of tiles: neighbouring
+// tiles, which share halo lines, then share one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    if ((nb & 7) != 0) return b;
+    return (b & 7) * (nb >> 3) + (b >> 3);
+}
 // batched launch: per-instance step arguments and first-tile offsets in device memory
 struct BatchLaunch {
     const void* args;      // StepArgs[ninst] or StepArgs32[ninst]

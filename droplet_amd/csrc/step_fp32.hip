@@ -41,11 +41,6 @@ __device__ __forceinline__ int wrap32(int v, int n, bool mod) {
     return v < 0 ? v + n : (v >= n ? v - n : v);
 }
 
-__device__ __forceinline__ int xcd_tile32(int b, int nb) {
-    if ((nb & 7) != 0) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
 __device__ __forceinline__ void step_tile_fp32(const StepArgs32& a, int tile) {
     constexpr int NN = TX * TY * TZ, NT = 2 * NN;
@@ -369,7 +364,7 @@ template <int NT> constexpr int waves_per_eu32() { return NT >= 1024 ? 8 : 6; }
 // one lattice: workgroup id -> tile (XCD-contiguous)
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
 __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu32<2 * TX * TY * TZ>()))) void step_fp32(StepArgs32 a) {
-    step_tile_fp32<TX, TY, TZ, WALL, FORCE, TAUMODE>(a, xcd_tile32(blockIdx.x, gridDim.x));
+    step_tile_fp32<TX, TY, TZ, WALL, FORCE, TAUMODE>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // many independent lattices in one launch: tile_start[i] is the first global
@@ -377,7 +372,7 @@ __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(
 // workgroup is found by binary search, its arguments read from global memory
 template <int TX, int TY, int TZ, bool WALL, int FORCE, int TAUMODE>
 __global__ __launch_bounds__(2 * TX* TY* TZ) __attribute__((amdgpu_waves_per_eu(waves_per_eu32<2 * TX * TY * TZ>()))) void step_batch_fp32(const StepArgs32* __restrict__ args, const int* __restrict__ tile_start, int ninst) {
-    const int b = xcd_tile32(blockIdx.x, gridDim.x);
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
     int lo = 0, hi = ninst - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;

@@ -42,14 +42,6 @@ __device__ __forceinline__ int wrapi(int v, int n, bool mod) {
     return v < 0 ? v + n : (v >= n ? v - n : v);
 }
 
-// Map the linear workgroup id to a tile so that each XCD (workgroups are dealt
-// round-robin to the 8 XCDs) gets a contiguous block of tiles: neighbouring
-// tiles, which share halo lines, then share one L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-    if ((nb & 7) != 0) return b;
-    return (b & 7) * (nb >> 3) + (b >> 3);
-}
-
 
 // tau_g: constant; from nu(phi); read from a field (strain Smagorinsky, computed
 // after each step); from nu(phi) and the stored sqrt(Pi:Pi) of the node's

@@ -25,6 +25,10 @@
     python -m droplet_amd.probe adjoint [--grid 256] [--steps 20] [--checkpoint 10]
         taped FP64 run of the shear droplet and the checkpointed adjoint sweep of
         J = D + A: ms per forward and per reverse step
+    python -m droplet_amd.probe noneq|scale|budget [--grid 256] [--steps 20]
+        bench.py's shear droplet with the non-equilibrium Smagorinsky model, with
+        scale-separated phi advection, or stepped through drp_step_energy_budget:
+        ms per step (and per budget call)
     python -m droplet_amd.probe ensemble [--members 64] [--steps 60] [--fp64]
         64 shear-droplet members, N uniform in {48, 64, 96, 128}, radius N/5,
         Ca uniform in [0.05, 0.5], centre offsets within +-2 cells
@@ -185,6 +189,38 @@ def adjoint(a, dl):
     lat.close()
 
 
+def variants(a, dl):
+    """bench.py's 256^3 shear droplet (FP64) in one of the model variants: the
+    non-equilibrium Smagorinsky model (Cs = 0.17, Delta = 1, nu2 = 2 nu1), the
+    scale-separated advection of the shells int|k| < 3 from step 1, or the step
+    with the energy budget of energy_budget.f90 (ms per call of
+    drp_step_energy_budget against a plain step)."""
+    from bench import shear_config
+    from droplet_amd.abi import DRP_LES_NONEQ
+    cfg, drops, _ = shear_config(a.grid)
+    if a.what == "noneq":
+        cfg.les_model, cfg.les_cs, cfg.les_delta, cfg.nu2 = DRP_LES_NONEQ, 0.17, 1.0, 2.0 * cfg.nu1
+    elif a.what == "scale":
+        cfg.advect_k_high, cfg.advect_k_low, cfg.advect_start = 3, 0, 1
+    lat = dl.Lattice(cfg)
+    lat.init_droplets(drops, 1.0 / 3.0)
+    lat.step(3)
+    n = a.grid ** 3
+    out = {"workload": f"{a.what}: shear droplet {a.grid}^3", "steps": a.steps}
+    if a.what == "budget":
+        import time
+        lat.step_energy_budget()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            terms = lat.step_energy_budget()
+        out["budget_ms_per_call"] = (time.perf_counter() - t0) * 1e3 / a.steps
+        out["terms"] = [float(v) for v in terms]
+    ms = lat.time_steps(a.steps)
+    out.update({"ms_per_step": ms / a.steps, "MLUPS": n * a.steps / (ms * 1e-3) / 1e6})
+    print(json.dumps(out))
+    lat.close()
+
+
 def droplet_census(a, dl):
     """Census of the turbulent-droplet state (32 droplets of radius 10) after a
     few steps: droplet count, volume range and time per call (HIP sync included)."""
@@ -210,7 +246,7 @@ def droplet_census(a, dl):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["copy", "copysoa", "yardsticks", "step", "autotune", "ensemble", "hit", "census", "allen-cahn",
-                                     "spectra", "adjoint"])
+                                     "spectra", "adjoint", "noneq", "scale", "budget"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
@@ -242,6 +278,9 @@ def main(argv=None):
         return
     if a.what == "adjoint":
         adjoint(a, dl)
+        return
+    if a.what in ("noneq", "scale", "budget"):
+        variants(a, dl)
         return
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
