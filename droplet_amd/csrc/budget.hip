@@ -130,6 +130,7 @@ constexpr int BPART = 8;  // partial sums per workgroup: the seven terms and a p
 // arrays instead of 9.  G_XY is formed at each stencil node from grad phi there
 // with the expression pass 1 would use.
 constexpr int BWORK = 7;
+constexpr int64_t BPAD = 0;
 enum { W_K = 0, W_V = 1, W_GP = 4 };
 
 struct BudgetArgs {
@@ -141,7 +142,8 @@ struct BudgetArgs {
     const double *fx, *fy, *fz;                // body force field of the step, or null
     const double *sinx, *cosx, *sinz, *cosz;   // Taylor-Green tables of the step, or null
     double tg_amp;
-    double* work;     // [BWORK][n]
+    double* work;     // [BWORK][ws]
+    int64_t ws;       // stride of the work arrays: n plus BPAD
     double* partial;  // [blocks][BPART]
 };
 
@@ -171,10 +173,9 @@ template <int K0, int K1> __device__ __forceinline__ void block_partials(double 
 }
 
 // Workgroup brick of the budget passes: 64 x 4 columns (a wavefront per x row)
-// that march over BZC planes, so the z neighbours of a plane's nodes were read by
-// the same workgroup one or two iterations earlier and are still in L2; bricks are
-// dealt to the XCDs in contiguous runs (xcd_tile), so bricks adjacent in y share
-// their halo rows in one L2.
+// that march over BZC planes, keeping the planes their stencils need in an LDS
+// ring; bricks are dealt to the XCDs in contiguous runs (xcd_tile), so bricks
+// adjacent in y read their shared halo rows through one L2.
 constexpr int BTX = 64, BTY = 4, BZC = 16;
 
 __host__ __device__ inline int64_t budget_bricks(int nx, int ny, int nz) {
@@ -198,20 +199,114 @@ __device__ __forceinline__ Brick brick(const BGrid& G) {
     return b;
 }
 
-__global__ __launch_bounds__(256) void budget_pass1(BudgetArgs a) {
+// Three z-planes of NA arrays over a brick and its one-node halo in x and y, in
+// LDS: a ring that the brick's z-march refills one plane at a time, so that each
+// value is read from memory once per brick rather than once per stencil use.
+constexpr int RX = BTX + 2, RY = BTY + 2, RN = RX * RY;
+template <int NA> constexpr int ring_per_thread() { return (NA * RN + 255) / 256; }
+
+__device__ __forceinline__ int wrap_any(int v, int n) {  // v in [-1, n + BTX]
+    v = v < 0 ? v + n : v;
+    return v >= n ? v % n : v;
+}
+
+// plane z of the NA arrays src(k) over the brick at (x0, y0) and its halo, into
+// registers (ring_per_thread<NA>() per thread)
+template <int NA, typename Src>
+__device__ __forceinline__ void ring_fetch(const BGrid& G, int x0, int y0, int z, Src&& src,
+                                           double (&r)[ring_per_thread<NA>()]) {
+    const int64_t zo = (int64_t)G.nx * G.ny * wrap_any(z, G.nz);
+#pragma unroll
+    for (int j = 0; j < ring_per_thread<NA>(); j++) {
+        const int i = (int)threadIdx.x + 256 * j;
+        if (i < NA * RN) {
+            const int k = i / RN, c = i - k * RN, ly = c / RX, lx = c - ly * RX;
+            const int gx = wrap_any(x0 - 1 + lx, G.nx), gy = wrap_any(y0 - 1 + ly, G.ny);
+            r[j] = src(k)[zo + gx + (int64_t)G.nx * gy];
+        }
+    }
+}
+template <int NA> __device__ __forceinline__ void ring_store(double* slot, const double (&r)[ring_per_thread<NA>()]) {
+#pragma unroll
+    for (int j = 0; j < ring_per_thread<NA>(); j++) {
+        const int i = (int)threadIdx.x + 256 * j;
+        if (i < NA * RN) slot[i] = r[j];
+    }
+}
+
+// lap_cal and grad_cal of phi and grad of u (gu[k][a] = d_a u_k) from a ring of
+// the old phi, u1, u2, u3 (the order and arithmetic of stencils())
+__device__ __forceinline__ void stencils_ring(const double* const (&sl)[3], double& lap, double gp[3],
+                                              double gu[3][3]) {
+    lap = -14.0 * sl[1][0];
+    for (int a = 0; a < 3; a++) {
+        gp[a] = 0.0;
+        gu[0][a] = gu[1][a] = gu[2][a] = 0.0;
+    }
+    sfor<1, 15>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        using D = Dir<q>;
+        const double* t = sl[1 + D::cz] + (D::cy * RX + D::cx);
+        const double v = t[0];
+        lap = lap + v;
+        gp[0] = acc<D::cx>(gp[0], v);
+        gp[1] = acc<D::cy>(gp[1], v);
+        gp[2] = acc<D::cz>(gp[2], v);
+        const double w[3] = {t[RN], t[2 * RN], t[3 * RN]};
+        for (int k = 0; k < 3; k++) {
+            gu[k][0] = acc<D::cx>(gu[k][0], w[k]);
+            gu[k][1] = acc<D::cy>(gu[k][1], w[k]);
+            gu[k][2] = acc<D::cz>(gu[k][2], w[k]);
+        }
+        // pin the sums after each neighbour (otherwise every neighbour's reads
+        // are issued first and held)
+        asm volatile("" : "+v"(lap), "+v"(gp[0]), "+v"(gp[1]), "+v"(gp[2]));
+#pragma unroll
+        for (int k = 0; k < 3; k++) asm volatile("" : "+v"(gu[k][0]), "+v"(gu[k][1]), "+v"(gu[k][2]));
+    });
+    lap = lap / 5.0;
+    for (int a = 0; a < 3; a++) {
+        gp[a] = gp[a] / 10.0;
+        for (int k = 0; k < 3; k++) gu[k][a] = gu[k][a] / 10.0;
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void budget_pass1(BudgetArgs a) {
     const BGrid& G = a.G;
     const Phys& P = a.P;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     const Brick b = brick(G);
+    // ring of the old phi, u1, u2, u3
+    constexpr int NA1 = 4;
+    __shared__ double ring[3][NA1 * RN];
+    const int lx = (int)threadIdx.x % BTX, ly = (int)threadIdx.x / BTX;
+    const int x0 = b.x - lx, y0 = b.y - ly;
+    const auto src = [&](int k) -> const double* { return a.old_mac + (k == 0 ? 0 : (k + 1) * n); };
+    double r[ring_per_thread<NA1>()];
+    ring_fetch<NA1>(G, x0, y0, b.z0 - 1, src, r);
+    ring_store<NA1>(ring[0], r);
+    ring_fetch<NA1>(G, x0, y0, b.z0, src, r);
+    ring_store<NA1>(ring[1], r);
+    ring_fetch<NA1>(G, x0, y0, b.z0 + 1, src, r);
+    const int me = (ly + 1) * RX + (lx + 1);
     double v[BPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int z = b.z0; b.live && z < b.z1; z++) {
+    for (int z = b.z0; z < b.z1; z++) {
+        // planes z - 1, z, z + 1 sit in slots j, j + 1, j + 2 (mod 3)
+        const int j = z - b.z0;
+        ring_store<NA1>(ring[(j + 2) % 3], r);
+        __syncthreads();
+        if (z + 2 <= b.z1) ring_fetch<NA1>(G, x0, y0, z + 2, src, r);
+        if (!b.live) {
+            __syncthreads();
+            continue;
+        }
         const int x = b.x, y = b.y;
         const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
-        const double* om = a.old_mac;
         const double* nm = a.new_mac;
+        const double* sl[3] = {ring[j % 3] + me, ring[(j + 1) % 3] + me, ring[(j + 2) % 3] + me};
         double lap, gp[3], gu[3][3];
-        stencils(G, om, om + 2 * n, om + 3 * n, om + 4 * n, x, y, z, lap, gp, gu);
-        const double ph = om[s];
+        stencils_ring(sl, lap, gp, gu);
+        const double ph = sl[1][0];
         const double p0 = ph * P.T / (1.0 - P.b * ph) - P.a * (ph * ph);
         const double gsq = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
         const double phn = nm[s];
@@ -238,74 +333,97 @@ __global__ __launch_bounds__(256) void budget_pass1(BudgetArgs a) {
         }
         v[6] += un[0] * F[0] + un[1] * F[1] + un[2] * F[2];
         double* w = a.work;
-        w[W_K * n + s] = 0.5 * (un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
-        for (int Y = 0; Y < 3; Y++) w[(W_V + Y) * n + s] = un[0] * S[0][Y] + un[1] * S[1][Y] + un[2] * S[2][Y];
-        for (int X = 0; X < 3; X++) w[(W_GP + X) * n + s] = gp[X];
+        const int64_t ws = a.ws;
+        w[W_K * ws + s] = 0.5 * (un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
+        for (int Y = 0; Y < 3; Y++) w[(W_V + Y) * ws + s] = un[0] * S[0][Y] + un[1] * S[1][Y] + un[2] * S[2][Y];
+        for (int X = 0; X < 3; X++) w[(W_GP + X) * ws + s] = gp[X];
+        __syncthreads();
     }
     block_partials<0, BPART>(v, a.partial);
 }
+
+// ring arrays of pass 2: K, V_0..2, grad phi 0..2 (work space), p (new fields)
+constexpr int NA2 = 8;
+enum { R_K = 0, R_V = 1, R_GP = 4, R_P = 7 };
 
 __global__ __launch_bounds__(256) void budget_pass2(BudgetArgs a) {
     const BGrid& G = a.G;
     const Phys& P = a.P;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
     const Brick b = brick(G);
+    __shared__ double ring[3][NA2 * RN];
+    const int lx = (int)threadIdx.x % BTX, ly = (int)threadIdx.x / BTX;
+    const int x0 = b.x - lx, y0 = b.y - ly;
+    const auto src = [&](int k) -> const double* { return k < R_P ? a.work + k * a.ws : a.new_mac + n; };
+    double r[ring_per_thread<NA2>()];
+    ring_fetch<NA2>(G, x0, y0, b.z0 - 1, src, r);
+    ring_store<NA2>(ring[0], r);
+    ring_fetch<NA2>(G, x0, y0, b.z0, src, r);
+    ring_store<NA2>(ring[1], r);
+    ring_fetch<NA2>(G, x0, y0, b.z0 + 1, src, r);
     double v[BPART] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int z = b.z0; b.live && z < b.z1; z++) {
-        const int x = b.x, y = b.y;
-        const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
-        const double* nm = a.new_mac;
-        const double* w = a.work;
-        const double* p = nm + n;
-        const double* wK = w + W_K * n;
-        const double* wV[3] = {w + W_V * n, w + (W_V + 1) * n, w + (W_V + 2) * n};
-        const double* wg[3] = {w + W_GP * n, w + (W_GP + 1) * n, w + (W_GP + 2) * n};
-        // grad_cal sums (sum over the directions of c_A v(x + c), in direction
-        // order) of K, p and G_XA along every axis A and of V_A along A, in one
-        // pass over the 14 neighbours; G_XA is formed at the neighbour
-        double aK[3] = {0.0, 0.0, 0.0}, aP[3] = {0.0, 0.0, 0.0}, aV[3] = {0.0, 0.0, 0.0};
-        double aG[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
-        sfor<1, 15>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            using D = Dir<q>;
-            const uint32_t t = G.off8(x + D::cx, y + D::cy, z + D::cz);
-            const double Kt = ld8(wK, t), pt = ld8(p, t);
-            const double gt[3] = {ld8(wg[0], t), ld8(wg[1], t), ld8(wg[2], t)};
-            const double gsq = gt[0] * gt[0] + gt[1] * gt[1] + gt[2] * gt[2];
-            sfor<0, 3>([&](auto Ac) {
-                constexpr int A = decltype(Ac)::value;
-                constexpr int c = cdir<q, A>();
-                if constexpr (c != 0) {
-                    aK[A] = acc<c>(aK[A], Kt);
-                    aP[A] = acc<c>(aP[A], pt);
-                    aV[A] = acc<c>(aV[A], ld8(wV[A], t));
-                    sfor<0, 3>([&](auto Xc) {
-                        constexpr int X = decltype(Xc)::value;
-                        aG[X][A] = acc<c>(aG[X][A], 4.5 * gt[X] * gt[A] - 1.5 * gsq * (X == A ? 1.0 : 0.0));
-                    });
+    const int me = (ly + 1) * RX + (lx + 1);
+    for (int z = b.z0; z < b.z1; z++) {
+        // planes z - 1, z, z + 1 sit in slots j, j + 1, j + 2 (mod 3)
+        const int j = z - b.z0;
+        ring_store<NA2>(ring[(j + 2) % 3], r);
+        __syncthreads();
+        if (z + 2 <= b.z1) ring_fetch<NA2>(G, x0, y0, z + 2, src, r);
+        if (b.live) {
+            const double* sl[3] = {ring[j % 3] + me, ring[(j + 1) % 3] + me, ring[(j + 2) % 3] + me};
+            // grad_cal sums (sum over the directions of c_A v(x + c), in direction
+            // order) of K, p and G_XA along every axis A and of V_A along A, in one
+            // pass over the 14 neighbours; G_XA is formed at the neighbour
+            double aK[3] = {0.0, 0.0, 0.0}, aP[3] = {0.0, 0.0, 0.0}, aV[3] = {0.0, 0.0, 0.0};
+            double aG[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+            sfor<1, 15>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                using D = Dir<q>;
+                const double* t = sl[1 + D::cz] + (D::cy * RX + D::cx);
+                const double Kt = t[R_K * RN], pt = t[R_P * RN];
+                const double gt[3] = {t[R_GP * RN], t[(R_GP + 1) * RN], t[(R_GP + 2) * RN]};
+                const double gsq = gt[0] * gt[0] + gt[1] * gt[1] + gt[2] * gt[2];
+                // G_XA = 4.5 g_X g_A - 1.5 |g|^2 delta_XA; off the diagonal the
+                // reference subtracts (1.5 |g|^2) * 0 = +0, which changes no finite value
+                const double h[3] = {4.5 * gt[0], 4.5 * gt[1], 4.5 * gt[2]};
+                const double gsq15 = 1.5 * gsq;
+                sfor<0, 3>([&](auto Ac) {
+                    constexpr int A = decltype(Ac)::value;
+                    constexpr int c = cdir<q, A>();
+                    if constexpr (c != 0) {
+                        aK[A] = acc<c>(aK[A], Kt);
+                        aP[A] = acc<c>(aP[A], pt);
+                        aV[A] = acc<c>(aV[A], t[(R_V + A) * RN]);
+                        sfor<0, 3>([&](auto Xc) {
+                            constexpr int X = decltype(Xc)::value;
+                            const double GXA = X == A ? h[X] * gt[A] - gsq15 : h[X] * gt[A];
+                            aG[X][A] = acc<c>(aG[X][A], GXA);
+                        });
+                    }
+                });
+                // pin the sums after each neighbour: left alone, the compiler reads
+                // all 14 neighbours first and holds them (over 300 VGPRs)
+#pragma unroll
+                for (int A = 0; A < 3; A++) {
+                    asm volatile("" : "+v"(aK[A]), "+v"(aP[A]), "+v"(aV[A]));
+                    asm volatile("" : "+v"(aG[0][A]), "+v"(aG[1][A]), "+v"(aG[2][A]));
                 }
             });
-            // Pin the sums after each neighbour, and fence memory after every
-            // second one: left alone, the compiler issues all 14 neighbours'
-            // loads first and holds them (over 300 VGPRs, one wave per SIMD).
-#pragma unroll
-            for (int A = 0; A < 3; A++) {
-                asm volatile("" : "+v"(aK[A]), "+v"(aP[A]), "+v"(aV[A]));
-                asm volatile("" : "+v"(aG[0][A]), "+v"(aG[1][A]), "+v"(aG[2][A]));
+            const int64_t s = b.x + (int64_t)G.nx * (b.y + (int64_t)G.ny * z);
+            const double* nm = a.new_mac;
+            const double u[3] = {nm[2 * n + s], nm[3 * n + s], nm[4 * n + s]};
+            v[1] += u[0] * (aK[0] / 10.0) + u[1] * (aK[1] / 10.0) + u[2] * (aK[2] / 10.0);
+            const double r1 = u[0] * (aP[0] / 10.0) + u[1] * (aP[1] / 10.0) + u[2] * (aP[2] / 10.0);
+            v[2] += -1.0 * r1;
+            double r2 = 0.0, r4 = 0.0;
+            for (int Y = 0; Y < 3; Y++) {
+                r2 = r2 + aV[Y] / 10.0;
+                for (int X = 0; X < 3; X++) r4 = r4 + u[X] * (aG[X][Y] / 10.0);
             }
-            if constexpr (q % 2 == 0) asm volatile("" ::: "memory");
-        });
-        const double u[3] = {nm[2 * n + s], nm[3 * n + s], nm[4 * n + s]};
-        v[1] += u[0] * (aK[0] / 10.0) + u[1] * (aK[1] / 10.0) + u[2] * (aK[2] / 10.0);
-        const double r1 = u[0] * (aP[0] / 10.0) + u[1] * (aP[1] / 10.0) + u[2] * (aP[2] / 10.0);
-        v[2] += -1.0 * r1;
-        double r2 = 0.0, r4 = 0.0;
-        for (int Y = 0; Y < 3; Y++) {
-            r2 = r2 + aV[Y] / 10.0;
-            for (int X = 0; X < 3; X++) r4 = r4 + u[X] * (aG[X][Y] / 10.0);
+            v[3] += r2 * 2.0 * nu_new(a, nm[s]);
+            v[5] += r4 * (-1.0) * (2.0 / 9.0) * P.kg;
         }
-        v[3] += r2 * 2.0 * nu_new(a, nm[s]);
-        v[5] += r4 * (-1.0) * (2.0 / 9.0) * P.kg;
+        __syncthreads();
     }
     // pass 1 wrote columns 0, 4, 6 (and zeros in the others); add this pass's columns
     __shared__ double sh[BPART][4];
@@ -372,7 +490,7 @@ extern "C" int drp_step_energy_budget(drp_lattice* L, double* terms) {
     if (n >= ((int64_t)1 << 29)) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: grids of 2^29 or more nodes");
     const int64_t nb = budget_bricks(c.nx, c.ny, c.nz);
     if (!L->budget) {
-        HIP_TRYB(hipMalloc((void**)&L->budget, ((size_t)BWORK * n + (size_t)nb * BPART) * sizeof(double)));
+        HIP_TRYB(hipMalloc((void**)&L->budget, ((size_t)BWORK * (n + BPAD) + (size_t)nb * BPART) * sizeof(double)));
     }
     // the body force the step will use (its pointers do not depend on the ping-pong parity)
     const StepArgs sa = make_step_args(L, L->step + 1);
@@ -396,7 +514,8 @@ extern "C" int drp_step_energy_budget(drp_lattice* L, double* terms) {
         a.tg_amp = sa.tg_amp;
     }
     a.work = L->budget;
-    a.partial = L->budget + (size_t)BWORK * n;
+    a.ws = n + BPAD;
+    a.partial = L->budget + (size_t)BWORK * a.ws;
     hipLaunchKernelGGL(budget_pass1, dim3((unsigned)nb), dim3(256), 0, L->stream, a);
     HIP_TRYB(hipGetLastError());
     hipLaunchKernelGGL(budget_pass2, dim3((unsigned)nb), dim3(256), 0, L->stream, a);
