@@ -130,7 +130,6 @@ constexpr int BPART = 8;  // partial sums per workgroup: the seven terms and a p
 // arrays instead of 9.  G_XY is formed at each stencil node from grad phi there
 // with the expression pass 1 would use.
 constexpr int BWORK = 7;
-constexpr int64_t BPAD = 0;
 enum { W_K = 0, W_V = 1, W_GP = 4 };
 
 struct BudgetArgs {
@@ -143,7 +142,7 @@ struct BudgetArgs {
     const double *sinx, *cosx, *sinz, *cosz;   // Taylor-Green tables of the step, or null
     double tg_amp;
     double* work;     // [BWORK][ws]
-    int64_t ws;       // stride of the work arrays: n plus BPAD
+    int64_t ws;       // stride of the work arrays (n)
     double* partial;  // [blocks][BPART]
 };
 
@@ -490,7 +489,7 @@ extern "C" int drp_step_energy_budget(drp_lattice* L, double* terms) {
     if (n >= ((int64_t)1 << 29)) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: grids of 2^29 or more nodes");
     const int64_t nb = budget_bricks(c.nx, c.ny, c.nz);
     if (!L->budget) {
-        HIP_TRYB(hipMalloc((void**)&L->budget, ((size_t)BWORK * (n + BPAD) + (size_t)nb * BPART) * sizeof(double)));
+        HIP_TRYB(hipMalloc((void**)&L->budget, ((size_t)BWORK * n + (size_t)nb * BPART) * sizeof(double)));
     }
     // the body force the step will use (its pointers do not depend on the ping-pong parity)
     const StepArgs sa = make_step_args(L, L->step + 1);
@@ -514,7 +513,7 @@ extern "C" int drp_step_energy_budget(drp_lattice* L, double* terms) {
         a.tg_amp = sa.tg_amp;
     }
     a.work = L->budget;
-    a.ws = n + BPAD;
+    a.ws = n;
     a.partial = L->budget + (size_t)BWORK * a.ws;
     hipLaunchKernelGGL(budget_pass1, dim3((unsigned)nb), dim3(256), 0, L->stream, a);
     HIP_TRYB(hipGetLastError());
