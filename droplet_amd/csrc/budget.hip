@@ -295,47 +295,47 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
         ring_store<NA1>(ring[(j + 2) % 3], r);
         __syncthreads();
         if (z + 2 <= b.z1) ring_fetch<NA1>(G, x0, y0, z + 2, src, r);
-        if (!b.live) {
-            __syncthreads();
-            continue;
+        // every thread reaches each barrier at the same point: a barrier inside the
+        // branch would run twice in a wavefront whose lanes take both paths
+        if (b.live) {
+            const int x = b.x, y = b.y;
+            const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
+            const double* nm = a.new_mac;
+            const double* sl[3] = {ring[j % 3] + me, ring[(j + 1) % 3] + me, ring[(j + 2) % 3] + me};
+            double lap, gp[3], gu[3][3];
+            stencils_ring(sl, lap, gp, gu);
+            const double ph = sl[1][0];
+            const double p0 = ph * P.T / (1.0 - P.b * ph) - P.a * (ph * ph);
+            const double gsq = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
+            const double phn = nm[s];
+            const double un[3] = {nm[2 * n + s], nm[3 * n + s], nm[4 * n + s]};
+            // left1 (kinetic_former with phi_procs already the new phi)
+            const double former = p0 - P.kg * phn * lap - 1.0 / 6.0 * P.kg * gsq;
+            v[0] += fabs(nm[n + s] - former);
+            double S[3][3];
+            for (int bb = 0; bb < 3; bb++)
+                for (int c = 0; c < 3; c++) S[c][bb] = 0.5 * (gu[c][bb] + gu[bb][c]);
+            const double nu = nu_new(a, phn);
+            double r3 = 0.0;
+            for (int bb = 0; bb < 3; bb++)
+                for (int c = 0; c < 3; c++) r3 = r3 + S[c][bb] * S[c][bb];
+            v[4] += r3 * (-2.0) * nu;
+            double F[3] = {0.0, 0.0, 0.0};
+            if (a.fx) {
+                F[0] = a.fx[s];
+                F[1] = a.fy[s];
+                F[2] = a.fz[s];
+            } else if (a.sinx) {
+                F[0] = a.tg_amp * a.sinx[x] * a.cosz[z];
+                F[2] = -(a.tg_amp * a.cosx[x] * a.sinz[z]);
+            }
+            v[6] += un[0] * F[0] + un[1] * F[1] + un[2] * F[2];
+            double* w = a.work;
+            const int64_t ws = a.ws;
+            w[W_K * ws + s] = 0.5 * (un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
+            for (int Y = 0; Y < 3; Y++) w[(W_V + Y) * ws + s] = un[0] * S[0][Y] + un[1] * S[1][Y] + un[2] * S[2][Y];
+            for (int X = 0; X < 3; X++) w[(W_GP + X) * ws + s] = gp[X];
         }
-        const int x = b.x, y = b.y;
-        const int64_t s = x + (int64_t)G.nx * (y + (int64_t)G.ny * z);
-        const double* nm = a.new_mac;
-        const double* sl[3] = {ring[j % 3] + me, ring[(j + 1) % 3] + me, ring[(j + 2) % 3] + me};
-        double lap, gp[3], gu[3][3];
-        stencils_ring(sl, lap, gp, gu);
-        const double ph = sl[1][0];
-        const double p0 = ph * P.T / (1.0 - P.b * ph) - P.a * (ph * ph);
-        const double gsq = gp[0] * gp[0] + gp[1] * gp[1] + gp[2] * gp[2];
-        const double phn = nm[s];
-        const double un[3] = {nm[2 * n + s], nm[3 * n + s], nm[4 * n + s]};
-        // left1 (kinetic_former with phi_procs already the new phi)
-        const double former = p0 - P.kg * phn * lap - 1.0 / 6.0 * P.kg * gsq;
-        v[0] += fabs(nm[n + s] - former);
-        double S[3][3];
-        for (int bb = 0; bb < 3; bb++)
-            for (int c = 0; c < 3; c++) S[c][bb] = 0.5 * (gu[c][bb] + gu[bb][c]);
-        const double nu = nu_new(a, phn);
-        double r3 = 0.0;
-        for (int bb = 0; bb < 3; bb++)
-            for (int c = 0; c < 3; c++) r3 = r3 + S[c][bb] * S[c][bb];
-        v[4] += r3 * (-2.0) * nu;
-        double F[3] = {0.0, 0.0, 0.0};
-        if (a.fx) {
-            F[0] = a.fx[s];
-            F[1] = a.fy[s];
-            F[2] = a.fz[s];
-        } else if (a.sinx) {
-            F[0] = a.tg_amp * a.sinx[x] * a.cosz[z];
-            F[2] = -(a.tg_amp * a.cosx[x] * a.sinz[z]);
-        }
-        v[6] += un[0] * F[0] + un[1] * F[1] + un[2] * F[2];
-        double* w = a.work;
-        const int64_t ws = a.ws;
-        w[W_K * ws + s] = 0.5 * (un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
-        for (int Y = 0; Y < 3; Y++) w[(W_V + Y) * ws + s] = un[0] * S[0][Y] + un[1] * S[1][Y] + un[2] * S[2][Y];
-        for (int X = 0; X < 3; X++) w[(W_GP + X) * ws + s] = gp[X];
         __syncthreads();
     }
     block_partials<0, BPART>(v, a.partial);

@@ -105,9 +105,14 @@ def test_phi_contribution_vs_numpy(shape):
 
 
 def _budget_case(case):
-    shape = (18, 20, 22)
+    # odd_bricks: the budget kernels work 64 x 4 x 16 bricks; nx = 70, ny = 7 and
+    # nz = 5 leave partial bricks in every direction and halos that wrap by modulo
+    shape = (5, 7, 70) if case == "odd_bricks" else (18, 20, 22)
     nz, ny, nx = shape
-    if case == "periodic_tg":
+    if case == "odd_bricks":
+        cfg = make_config(nx, ny, nz, kappa_g=0.01, nu1=0.05, nu2=0.12)
+        force = None
+    elif case == "periodic_tg":
         cfg = make_config(nx, ny, nz, kappa_g=0.01, nu1=0.05, nu2=0.12, force_kind=DRP_FORCE_TAYLOR_GREEN,
                           force_amp=2e-5, force_k=2 * math.pi / nx, force_start=3)
         force = None
@@ -119,7 +124,7 @@ def _budget_case(case):
     return cfg, shape, force
 
 
-@pytest.mark.parametrize("case", ["periodic_tg", "walls_field"])
+@pytest.mark.parametrize("case", ["periodic_tg", "walls_field", "odd_bricks"])
 def test_energy_budget_vs_restatement(case):
     from droplet_amd.lattice import Lattice
     cfg, shape, force = _budget_case(case)
