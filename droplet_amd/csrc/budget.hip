@@ -52,19 +52,7 @@ struct BGrid {
         z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
         return x + (int64_t)nx * (y + (int64_t)ny * z);
     }
-    // byte offset of a double at (x, y, z), wrapped; valid for grids of fewer than
-    // 2^29 nodes, where it fits 32 bits (loads then take a uniform base address
-    // and a 32-bit lane offset)
-    __device__ __forceinline__ uint32_t off8(int x, int y, int z) const {
-        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
-        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
-        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
-        return 8u * ((uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)z));
-    }
 };
-__device__ __forceinline__ double ld8(const double* base, uint32_t off) {
-    return *reinterpret_cast<const double*>(reinterpret_cast<const char*>(base) + off);
-}
 
 // component A of direction Q
 template <int Q, int A> constexpr int cdir() { return A == 0 ? Dir<Q>::cx : (A == 1 ? Dir<Q>::cy : Dir<Q>::cz); }
@@ -486,7 +474,6 @@ extern "C" int drp_step_energy_budget(drp_lattice* L, double* terms) {
     if (c.precision != DRP_FP64) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: FP64 lattices only");
     if (c.model != DRP_MODEL_FREE_ENERGY) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: free-energy model only");
     const int64_t n = L->n;
-    if (n >= ((int64_t)1 << 29)) return drp_fail(DRP_ERR_UNSUPPORTED, "energy budget: grids of 2^29 or more nodes");
     const int64_t nb = budget_bricks(c.nx, c.ny, c.nz);
     if (!L->budget) {
         HIP_TRYB(hipMalloc((void**)&L->budget, ((size_t)BWORK * n + (size_t)nb * BPART) * sizeof(double)));

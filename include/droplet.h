@@ -450,8 +450,8 @@ int drp_phi_contribution(drp_lattice* lat, double* C, int nshells, double* bands
  * (its active code never assigns K, so its left2 is always 0), and right3 is each
  * step's own term (the reference never resets its array, so its right3 accumulates
  * over the run).  right2 is evaluated as sum 2 nu sum_b d_b (sum_a u_a S_ab), equal to
- * the reference's form but for rounding.  FP64, free-energy model, fewer than 2^29
- * nodes; allocates 7 doubles per node on the first call. */
+ * the reference's form but for rounding.  FP64, free-energy model; allocates 7
+ * doubles per node on the first call. */
 int drp_step_energy_budget(drp_lattice* lat, double* terms);
 
 /* Message for the calling thread's last failure ("" if none). */
