@@ -120,6 +120,13 @@ constexpr int BPART = 8;  // partial sums per workgroup: the seven terms and a p
 constexpr int BWORK = 7;
 enum { W_K = 0, W_V = 1, W_GP = 4 };
 
+// Workgroup brick of the budget passes: 64 x 4 columns (a wavefront per x row)
+// that march over BZC planes, keeping the planes their stencils need in an LDS
+// ring; bricks are dealt to the XCDs in contiguous runs (xcd_tile), so bricks
+// adjacent in y read their shared halo rows through one L2.
+constexpr int BTX = 64, BTY = 4, BZC = 16;
+constexpr int BT = BTX * BTY, NW = BT / 64;  // threads and wavefronts per workgroup
+
 struct BudgetArgs {
     BGrid G;
     Phys P;
@@ -143,7 +150,7 @@ __device__ __forceinline__ double nu_new(const BudgetArgs& a, double phn) {
 
 // workgroup partial sums of v[k] for the columns in [k0, k1)
 template <int K0, int K1> __device__ __forceinline__ void block_partials(double (&v)[BPART], double* partial) {
-    __shared__ double sh[BPART][4];
+    __shared__ double sh[BPART][NW];
     for (int k = K0; k < K1; k++) {
         double t = v[k];
         for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
@@ -155,15 +162,12 @@ template <int K0, int K1> __device__ __forceinline__ void block_partials(double 
     __syncthreads();
     if (threadIdx.x >= K0 && threadIdx.x < K1) {
         const int k = threadIdx.x;
-        partial[blockIdx.x * BPART + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+        double t = sh[k][0];
+        for (int w = 1; w < NW; w++) t += sh[k][w];
+        partial[blockIdx.x * BPART + k] = t;
     }
 }
 
-// Workgroup brick of the budget passes: 64 x 4 columns (a wavefront per x row)
-// that march over BZC planes, keeping the planes their stencils need in an LDS
-// ring; bricks are dealt to the XCDs in contiguous runs (xcd_tile), so bricks
-// adjacent in y read their shared halo rows through one L2.
-constexpr int BTX = 64, BTY = 4, BZC = 16;
 
 __host__ __device__ inline int64_t budget_bricks(int nx, int ny, int nz) {
     return (int64_t)((nx + BTX - 1) / BTX) * ((ny + BTY - 1) / BTY) * ((nz + BZC - 1) / BZC);
@@ -190,7 +194,7 @@ __device__ __forceinline__ Brick brick(const BGrid& G) {
 // LDS: a ring that the brick's z-march refills one plane at a time, so that each
 // value is read from memory once per brick rather than once per stencil use.
 constexpr int RX = BTX + 2, RY = BTY + 2, RN = RX * RY;
-template <int NA> constexpr int ring_per_thread() { return (NA * RN + 255) / 256; }
+template <int NA> constexpr int ring_per_thread() { return (NA * RN + BT - 1) / BT; }
 
 __device__ __forceinline__ int wrap_any(int v, int n) {  // v in [-1, n + BTX]
     v = v < 0 ? v + n : v;
@@ -205,7 +209,7 @@ __device__ __forceinline__ void ring_fetch(const BGrid& G, int x0, int y0, int z
     const int64_t zo = (int64_t)G.nx * G.ny * wrap_any(z, G.nz);
 #pragma unroll
     for (int j = 0; j < ring_per_thread<NA>(); j++) {
-        const int i = (int)threadIdx.x + 256 * j;
+        const int i = (int)threadIdx.x + BT * j;
         if (i < NA * RN) {
             const int k = i / RN, c = i - k * RN, ly = c / RX, lx = c - ly * RX;
             const int gx = wrap_any(x0 - 1 + lx, G.nx), gy = wrap_any(y0 - 1 + ly, G.ny);
@@ -216,7 +220,7 @@ __device__ __forceinline__ void ring_fetch(const BGrid& G, int x0, int y0, int z
 template <int NA> __device__ __forceinline__ void ring_store(double* slot, const double (&r)[ring_per_thread<NA>()]) {
 #pragma unroll
     for (int j = 0; j < ring_per_thread<NA>(); j++) {
-        const int i = (int)threadIdx.x + 256 * j;
+        const int i = (int)threadIdx.x + BT * j;
         if (i < NA * RN) slot[i] = r[j];
     }
 }
@@ -258,7 +262,7 @@ __device__ __forceinline__ void stencils_ring(const double* const (&sl)[3], doub
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void budget_pass1(BudgetArgs a) {
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void budget_pass1(BudgetArgs a) {
     const BGrid& G = a.G;
     const Phys& P = a.P;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
@@ -333,7 +337,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void b
 constexpr int NA2 = 8;
 enum { R_K = 0, R_V = 1, R_GP = 4, R_P = 7 };
 
-__global__ __launch_bounds__(256) void budget_pass2(BudgetArgs a) {
+__global__ __launch_bounds__(BT) void budget_pass2(BudgetArgs a) {
     const BGrid& G = a.G;
     const Phys& P = a.P;
     const int64_t n = (int64_t)G.nx * G.ny * G.nz;
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(256) void budget_pass2(BudgetArgs a) {
         __syncthreads();
     }
     // pass 1 wrote columns 0, 4, 6 (and zeros in the others); add this pass's columns
-    __shared__ double sh[BPART][4];
+    __shared__ double sh[BPART][NW];
     for (int k = 1; k < 6; k++) {
         double t = v[k];
         for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
@@ -425,7 +429,9 @@ __global__ __launch_bounds__(256) void budget_pass2(BudgetArgs a) {
     __syncthreads();
     if (threadIdx.x == 1 || threadIdx.x == 2 || threadIdx.x == 3 || threadIdx.x == 5) {
         const int k = threadIdx.x;
-        a.partial[blockIdx.x * BPART + k] = ((sh[k][0] + sh[k][1]) + sh[k][2]) + sh[k][3];
+        double t = sh[k][0];
+        for (int w = 1; w < NW; w++) t += sh[k][w];
+        a.partial[blockIdx.x * BPART + k] = t;
     }
 }
 
@@ -502,9 +508,9 @@ extern "C" int drp_step_energy_budget(drp_lattice* L, double* terms) {
     a.work = L->budget;
     a.ws = n;
     a.partial = L->budget + (size_t)BWORK * a.ws;
-    hipLaunchKernelGGL(budget_pass1, dim3((unsigned)nb), dim3(256), 0, L->stream, a);
+    hipLaunchKernelGGL(budget_pass1, dim3((unsigned)nb), dim3(BT), 0, L->stream, a);
     HIP_TRYB(hipGetLastError());
-    hipLaunchKernelGGL(budget_pass2, dim3((unsigned)nb), dim3(256), 0, L->stream, a);
+    hipLaunchKernelGGL(budget_pass2, dim3((unsigned)nb), dim3(BT), 0, L->stream, a);
     HIP_TRYB(hipGetLastError());
     std::vector<double> part((size_t)nb * BPART);
     HIP_TRYB(hipMemcpyAsync(part.data(), a.partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, L->stream));
