@@ -10,13 +10,15 @@
 //                            fields it produced
 //
 // Neither is on the step's hot path (energy_budget.f90 evaluates the budget after
-// every step).  The budget runs in two passes over the nodes: the first forms the per-node terms that need no stencil of a derived
-// quantity (left1, right3, right5) and stores K = |u|^2/2, V_Y = u_X S_XY and
-// grad phi (7 doubles per node, allocated on the first call); the second takes
-// the D3Q15 gradients of those, of p and of the Korteweg tensor G_XY formed at
-// each stencil node (left2, right1, right2, right4).  Each workgroup leaves its partial sums; the host adds them with
-// Neumaier compensation in workgroup order.  Every per-node expression keeps
-// the reference's evaluation order (FMA contraction is off for this file).
+// every step).  The budget runs in two passes over the nodes: the first forms the
+// per-node terms that need no stencil of a derived quantity (left1, right3,
+// right5) and stores K = |u|^2/2, V_Y = u_X S_XY and grad phi (7 doubles per
+// node, allocated on the first call); the second takes the D3Q15 gradients of
+// those, of p and of the Korteweg tensor G_XY formed at each stencil node (left2,
+// right1, right2, right4).  Each workgroup leaves its partial sums; the host adds
+// them with Neumaier compensation in workgroup order.  The per-node expressions
+// keep the reference's evaluation order (FMA contraction is off for this file),
+// except that right2 sums d_Y V_Y rather than d_Y (u_X S_XY) term by term.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
