@@ -17,7 +17,8 @@ import math
 import numpy as np
 import pytest
 
-from droplet_amd.abi import (DRP_FORCE_FIELD, DRP_FORCE_TAYLOR_GREEN, DRP_FP32_SHIFTED, DRP_WALL_Y, make_config)
+from droplet_amd.abi import (DRP_FORCE_FIELD, DRP_FORCE_TAYLOR_GREEN, DRP_FP32_SHIFTED, DRP_WALL_Y, DRP_WALL_Z,
+                             make_config)
 from oracle import pyoracle
 from oracle.pyoracle import OracleLattice
 
@@ -112,6 +113,10 @@ def _budget_case(case):
     if case == "odd_bricks":
         cfg = make_config(nx, ny, nz, kappa_g=0.01, nu1=0.05, nu2=0.12)
         force = None
+    elif case == "walls_z":  # moving walls normal to z (dropch.f90), nu(phi) varying
+        cfg = make_config(nx, ny, nz, kappa_g=0.01, nu1=0.05, nu2=0.12, wall_axis=DRP_WALL_Z, wall_u=0.02,
+                          wall_start=1)
+        force = None
     elif case == "periodic_tg":
         cfg = make_config(nx, ny, nz, kappa_g=0.01, nu1=0.05, nu2=0.12, force_kind=DRP_FORCE_TAYLOR_GREEN,
                           force_amp=2e-5, force_k=2 * math.pi / nx, force_start=3)
@@ -124,7 +129,7 @@ def _budget_case(case):
     return cfg, shape, force
 
 
-@pytest.mark.parametrize("case", ["periodic_tg", "walls_field", "odd_bricks"])
+@pytest.mark.parametrize("case", ["periodic_tg", "walls_field", "odd_bricks", "walls_z"])
 def test_energy_budget_vs_restatement(case):
     from droplet_amd.lattice import Lattice
     cfg, shape, force = _budget_case(case)
