@@ -29,6 +29,10 @@
         bench.py's shear droplet with the non-equilibrium Smagorinsky model, with
         scale-separated phi advection, or stepped through drp_step_energy_budget:
         ms per step (and per budget call)
+    python -m droplet_amd.probe diag [--grid 256] [--reps 20]
+        ms per drp_reduce, drp_deformation and copy of the five fields to the host,
+        per write of the reference's field files at grid^3 and per write and
+        read-back of its population (restart) files at (grid/2)^3
     python -m droplet_amd.probe ensemble [--members 64] [--steps 60] [--fp64]
         64 shear-droplet members, N uniform in {48, 64, 96, 128}, radius N/5,
         Ca uniform in [0.05, 0.5], centre offsets within +-2 cells
@@ -221,6 +225,54 @@ def variants(a, dl):
     lat.close()
 
 
+def diagnostics(a, dl):
+    """Analysis hooks and file formats on bench.py's shear droplet (FP64): ms per
+    drp_reduce, drp_deformation and drp_get_macro of the five fields; the
+    reference-format field files (output) at grid^3 and a population restart
+    (outputfg written, then read back) at (grid/2)^3, in a temporary directory."""
+    import shutil
+    import tempfile
+    import time
+    from bench import shear_config
+    from droplet_amd.abi import DRP_FILE_FG, DRP_FILE_FIELDS
+
+    def per_call(fn, reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) * 1e3 / reps
+
+    out = {"grid": a.grid}
+    cfg, drops, _ = shear_config(a.grid)
+    lat = dl.Lattice(cfg)
+    lat.init_droplets(drops, 1.0 / 3.0)
+    lat.step(3)
+    out["reduce_ms"] = per_call(lat.reduce, a.reps)
+    out["deformation_ms"] = per_call(lambda: lat.deformation(0), a.reps)
+    out["fields_to_host_ms"] = per_call(lat.fields, a.reps)
+    d = tempfile.mkdtemp(prefix="drp_files_")
+    try:
+        out["write_fields_ms"] = per_call(lambda: lat.write_reference_files(d, 3, DRP_FILE_FIELDS), 2)
+        out["fields_file_bytes"] = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        lat.close()
+        shutil.rmtree(d)
+        os.makedirs(d)
+        g = a.grid // 2
+        cfg, drops, _ = shear_config(g)
+        lat = dl.Lattice(cfg)
+        lat.init_droplets(drops, 1.0 / 3.0)
+        lat.step(3)
+        out["restart_grid"] = g
+        out["write_fg_ms"] = per_call(lambda: lat.write_reference_files(d, 3, DRP_FILE_FG), 1)
+        out["fg_file_bytes"] = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        out["read_fg_ms"] = per_call(lambda: lat.read_reference_files(d, 3, DRP_FILE_FG), 1)
+        lat.close()
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    print(json.dumps(out))
+
+
 def droplet_census(a, dl):
     """Census of the turbulent-droplet state (32 droplets of radius 10) after a
     few steps: droplet count, volume range and time per call (HIP sync included)."""
@@ -246,7 +298,7 @@ def droplet_census(a, dl):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["copy", "copysoa", "yardsticks", "step", "autotune", "ensemble", "hit", "census", "allen-cahn",
-                                     "spectra", "adjoint", "noneq", "scale", "budget"])
+                                     "spectra", "adjoint", "noneq", "scale", "budget", "diag"])
     ap.add_argument("--mib", type=int, default=1024)
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--grid", type=int, default=256)
@@ -281,6 +333,9 @@ def main(argv=None):
         return
     if a.what in ("noneq", "scale", "budget"):
         variants(a, dl)
+        return
+    if a.what == "diag":
+        diagnostics(a, dl)
         return
     if a.what == "copy":
         gbs = dl.copy_bandwidth(a.mib << 20, a.reps)
